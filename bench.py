@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""Reinitialization throughput on one MI355X.
+
+Workload (default, "spheres3d"): 3D octree on the unit cube, min level 5,
+max level 10, refined where |phi0| <= 1.2 diag(cell); phi0 = min_i(|x-c_i| - r_i)
+* (1 + 0.5 sin(2 pi x) cos(2 pi y)) for 64 spheres (centres U[0.2,0.8]^3,
+radii U[0.03,0.08], seed 20261202).  One step = one reinitialize_2nd_order
+call (50 TVD-RK2 pseudo-time iterations) on phi0 already resident in HBM.
+
+Prints ONE JSON line on stdout (logs go to stderr):
+  value     mixed-precision node-iterations/s (FP32 bulk, FP64 subcell roots)
+  fp64      the plain FP64 path's node-iterations/s, and mixed/fp64
+  linf_err  L-inf |mixed - fp64| in the |phi| < 3 h_min band (FP64 GPU result,
+            bit-identical to the CPU oracle in the parity tests, is the reference)
+  roofline  dominant kernel: algorithmic bytes per launch / average launch
+            duration (HIP events on the launch stream) against 8 TB/s
+  cpu_baseline  the CPU oracle (C restatement of the reference arithmetic)
+            on this host's cores, on a bounded sample of the same workload
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "reinit node-iters/s on 1 GPU (mixed vs FP64 at <=1e-3h err); % of peak HBM BW"
+HBM_PEAK = 8.0e12
+SEED = 20261202
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def b_alg(dim, w):
+    """Fused algorithmic minimum bytes per node-iteration (TVD-RK2): per stage
+    read phi_in, read phi_0, write phi_out (3w) and one int32 neighbour index
+    per face (4*2*dim); plus one phi_n read for the RK2 average."""
+    return 2 * (3 * w + 8 * dim) + w
+
+
+def stage_bytes(dim, w):
+    return 3 * w + 8 * dim
+
+
+def dd_bytes(dim, w):
+    """pass A: read f, write dim second derivatives, one index per face."""
+    return w + dim * w + 8 * dim
+
+
+def build_workload(name, lmax_override=None):
+    import casl_p4est_amd as C
+    from casl_p4est_amd import levelset as ls
+    t0 = time.time()
+    if name == "spheres3d":
+        dim, lmin, lmax, iters = 3, 5, 10, 50
+        shape = ls.random_spheres(64, seed=SEED)
+        desc = ("3D octree unit cube, lmin 5 / lmax 10, |phi0|<=1.2 diag refinement, 64 spheres "
+                "(seed 20261202) x (1+0.5 sin 2pi x cos 2pi y), reinitialize_2nd_order 50 iterations")
+    elif name == "star2d":
+        dim, lmin, lmax, iters = 2, 4, 9, 20
+        shape = ls.Star()
+        desc = ("2D quadtree unit square, lmin 4 / lmax 9, star r=0.3+0.05cos5t x (1+0.8|x-c|^2), "
+                "reinitialize_2nd_order 20 iterations")
+    else:
+        raise ValueError(name)
+    if lmax_override:
+        lmax = lmax_override
+        desc += f" [lmax reduced to {lmax}]"
+    forest = C.Forest(dim).refine_levelset(shape, lmin, lmax, lip=2.4)
+    t1 = time.time()
+    forest.build_nodes()
+    t2 = time.time()
+    ngbd = C.NodeNeighbors(forest).init_neighbors()
+    t3 = time.time()
+    phi0 = forest.sample(shape)
+    log(f"[bench] host forest: refine {t1 - t0:.1f}s, nodes {t2 - t1:.1f}s, stencils {t3 - t2:.1f}s; "
+        f"{forest.num_nodes} nodes, {forest.num_leaves} leaves")
+    return dict(dim=dim, lmin=lmin, lmax=lmax, iters=iters, forest=forest, ngbd=ngbd, phi0=phi0, desc=desc)
+
+
+def time_steps(fn, steps, warmup, torch):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def kernel_profile(fn, torch):
+    from casl_p4est_amd import _lib as L
+    import ctypes
+    lib = L.lib()
+    lib.casl_profile_enable(1)
+    lib.casl_profile_reset()
+    fn()
+    torch.cuda.synchronize()
+    out = {}
+    for k in ("k_dd", "k_prep", "k_stage", "k_batch"):
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        L.call("casl_profile_query", k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        if cnt.value:
+            out[k] = (ms.value, cnt.value)
+    lib.casl_profile_enable(0)
+    return out
+
+
+def cpu_baseline(W, budget_s):
+    from oracle import pyoracle as O
+    tab, phi0, iters = W["ngbd"].tables, W["phi0"], W["iters"]
+    threads = O.num_threads()
+    t0 = time.perf_counter()
+    O.reinit(tab, phi0, 2, 2, 1)
+    t1 = time.perf_counter() - t0
+    n_it = 1
+    if t1 < budget_s / 3:
+        n_it = max(1, min(iters, int(budget_s / max(t1, 1e-3))))
+        t0 = time.perf_counter()
+        O.reinit(tab, phi0, 2, 2, n_it)
+        t1 = time.perf_counter() - t0
+    rate = tab.n_nodes * n_it / t1
+    return dict(value=rate, unit="node-iters/s", cores=threads, kind="port",
+                sample=f"full workload grid ({tab.n_nodes} nodes), {n_it} of {iters} RK2 iterations, FP64, "
+                       f"{threads} OpenMP threads, {t1:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d"])
+    ap.add_argument("--lmax", type=int, default=0, help="reduce max level (quick runs only)")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import casl_p4est_amd as C
+    if not torch.cuda.is_available():
+        raise RuntimeError("bench.py needs a HIP device")
+    dev = torch.device("cuda:0")
+    W = build_workload(args.config, args.lmax)
+    N, dim, iters = W["ngbd"].num_nodes, W["dim"], W["iters"]
+    h = float(np.min(W["forest"].dxyz_min()))
+    lsr = C.LevelSet(W["ngbd"])
+    W["ngbd"].device()  # upload tables before timing
+    phi0 = torch.from_numpy(W["phi0"]).to(dev)
+    phi = torch.empty_like(phi0)
+
+    def step(precision):
+        def run():
+            phi.copy_(phi0)
+            lsr.reinitialize_2nd_order(phi, iters, precision=precision)
+        return run
+
+    t_mixed = time_steps(step("mixed"), args.steps, args.warmup, torch)
+    out_mixed = phi.clone()
+    t_fp64 = time_steps(step("fp64"), args.steps, args.warmup, torch)
+    out_fp64 = phi.clone()
+    rate_mixed = N * iters * args.steps / t_mixed
+    rate_fp64 = N * iters * args.steps / t_fp64
+    err = W["ngbd"].device().band_linf(out_mixed, out_fp64, out_fp64, 3 * h)
+    log(f"[bench] mixed {rate_mixed:.3e} node-iters/s, fp64 {rate_fp64:.3e}, x{rate_mixed / rate_fp64:.2f}, "
+        f"Linf band err {err:.2e} = {err / h:.2e} h")
+
+    prof = {}
+    for prec, w in (("mixed", 4), ("fp64", 8)):
+        kp = kernel_profile(step(prec), torch)
+        kernels = {}
+        for k, (ms, cnt) in kp.items():
+            per = {"k_stage": stage_bytes(dim, w) * N, "k_dd": dd_bytes(dim, w) * N}.get(k)
+            avg = ms / cnt
+            kernels[k] = dict(avg_ms=avg, launches=cnt, total_ms=ms,
+                              alg_bytes_per_launch=per,
+                              achieved_GBps=(per / (avg * 1e-3) / 1e9) if per else None)
+        prof[prec] = kernels
+        log(f"[bench] kernels {prec}: " + ", ".join(f"{k} {v['avg_ms']:.3f} ms x{v['launches']}"
+                                                     for k, v in kernels.items()))
+    km = prof["mixed"]
+    dom = max((k for k in km if km[k]["alg_bytes_per_launch"]), key=lambda k: km[k]["total_ms"])
+    achieved = km[dom]["achieved_GBps"] * 1e9
+    roofline = dict(bound="hbm", kernel=dom, achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                    frac=achieved / HBM_PEAK, traffic=None,
+                    alg_bytes_per_launch=km[dom]["alg_bytes_per_launch"], avg_launch_ms=km[dom]["avg_ms"])
+    eff = dict(mixed=rate_mixed * b_alg(dim, 4) / HBM_PEAK, fp64=rate_fp64 * b_alg(dim, 8) / HBM_PEAK,
+               b_alg_mixed=b_alg(dim, 4), b_alg_fp64=b_alg(dim, 8))
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(W, args.cpu_budget)
+        log(f"[bench] cpu baseline {cpu['value']:.3e} node-iters/s ({cpu['sample']})")
+
+    result = {
+        "metric": METRIC,
+        "value": rate_mixed,
+        "unit": "node-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk + f64 subcell/norm",
+        "data": "synthetic",
+        "config": {"workload": W["desc"], "nodes": N, "iterations": iters, "h_min": h},
+        "fp64": {"value": rate_fp64, "unit": "node-iters/s", "ms_per_step": 1e3 * t_fp64 / args.steps},
+        "mixed_over_fp64": rate_mixed / rate_fp64,
+        "linf_err_band": err,
+        "linf_err_band_over_h": err / h,
+        "roofline": roofline,
+        "effective_hbm_frac": eff,
+        "kernels": prof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,73 @@
+"""Builds the native library libcasl_reinit.so in-tree for gfx950.
+
+forest.cpp (host C++) is compiled with g++, the HIP sources with hipcc for
+gfx950 only, and the objects are linked by hipcc into one shared library next
+to this file.  The build always recompiles (it is the build check), and
+writes objects under casl_p4est_amd/_build/.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libcasl_reinit.so")
+INCLUDE = os.path.join(os.path.dirname(HERE), "include")
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX_HOST", "g++")
+
+HIP_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    # strict IEEE evaluation: the FP64 path must reproduce the reference's
+    # operation order exactly
+    "-ffp-contract=off",
+    # no packed FP32 instructions
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-I", INCLUDE, "-I", CSRC,
+]
+HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", INCLUDE, "-I", CSRC]
+
+HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip"]
+HOST_SOURCES = ["forest.cpp"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build_library(verbose=True):
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    for src in HOST_SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        _run([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
+        objs.append(obj)
+    for src in HIP_SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(BUILD, src + ".o")
+        _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj], verbose)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(verbose=True):
+    oracle_dir = os.path.join(os.path.dirname(HERE), "oracle")
+    _run(["make", "-s", "-C", oracle_dir], verbose)
+    return os.path.join(oracle_dir, "liboracle.so")
+
+
+if __name__ == "__main__":
+    build_library(verbose=True)
+    if "--oracle" in sys.argv:
+        build_oracle(verbose=True)

@@ -1,0 +1,141 @@
+"""ctypes bindings of libcasl_reinit.so (the C ABI of include/casl_reinit.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if the shared library is missing, importing the bindings raises.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcasl_reinit.so")
+
+CASL_OK = 0
+CASL_ERR_INVALID_ARGUMENT = -1
+CASL_ERR_NO_DEVICE = -2
+CASL_ERR_HIP = -3
+CASL_ERR_OUT_OF_MEMORY = -4
+CASL_ERR_INEXACT_DISTANCES = -5
+CASL_ERR_INVALID_STENCIL = -6
+CASL_ERR_NOT_BUILT = -7
+
+CASL_PRECISION_FP64 = 0
+CASL_PRECISION_MIXED = 1
+
+CASL_LS_CALLBACK = 0
+CASL_LS_STAR = 1
+CASL_LS_SPHERES = 2
+CASL_LS_GYROID = 3
+CASL_LS_SPHERE = 4
+CASL_LS_PLANE = 5
+CASL_LS_MODULATED = 6
+
+c_int32_p = ctypes.POINTER(ctypes.c_int32)
+c_int64_p = ctypes.POINTER(ctypes.c_int64)
+c_double_p = ctypes.POINTER(ctypes.c_double)
+c_uint8_p = ctypes.POINTER(ctypes.c_uint8)
+c_int8_p = ctypes.POINTER(ctypes.c_int8)
+
+LEVELSET_FN = ctypes.CFUNCTYPE(ctypes.c_double, c_double_p, ctypes.c_void_p)
+
+
+class FlatStencil(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32),
+        ("n_nodes", ctypes.c_int64),
+        ("nbr", c_int32_p),
+        ("d", c_double_p),
+        ("d_tr", c_double_p),
+        ("wall", c_uint8_p),
+        ("zero_distance_threshold", ctypes.c_double),
+    ]
+
+
+class LevelSetSpec(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("n_items", ctypes.c_int32),
+        ("params", c_double_p),
+        ("fn", LEVELSET_FN),
+        ("ctx", ctypes.c_void_p),
+    ]
+
+
+class CaslError(RuntimeError):
+    """Non-zero status from the native library."""
+
+    def __init__(self, code, func, message):
+        super().__init__(f"{func} failed with status {code}: {message}")
+        self.code = code
+
+
+# The signatures exported by include/casl_reinit.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "casl_last_error": (ctypes.c_char_p, []),
+    "casl_stencil_create": (ctypes.c_int, [ctypes.POINTER(FlatStencil), ctypes.POINTER(ctypes.c_void_p)]),
+    "casl_stencil_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "casl_stencil_info": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p]),
+    "casl_second_derivatives": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "casl_reinit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),
+    "casl_reinit_masked": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),
+    "casl_band_linf": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                      ctypes.c_double, c_double_p, ctypes.c_void_p]),
+    "casl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "casl_profile_reset": (ctypes.c_int, []),
+    "casl_profile_query": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), c_int64_p]),
+    "casl_batch_create": (ctypes.c_int, [ctypes.POINTER(FlatStencil), ctypes.c_int32,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "casl_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "casl_batch_info": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int32_p, c_int32_p]),
+    "casl_batch_set_schedule": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "casl_batch_get_schedule": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p]),
+    "casl_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p, c_int32_p, c_int32_p, ctypes.POINTER(ctypes.c_float)]),
+    "casl_batch_reinit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_void_p]),
+    "casl_batch_reinit_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                               ctypes.c_void_p]),
+    "casl_levelset_eval": (ctypes.c_int, [ctypes.POINTER(LevelSetSpec), ctypes.c_int, ctypes.c_int64,
+                                          c_double_p, c_double_p]),
+    "casl_forest_new": (ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "casl_forest_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "casl_forest_refine_levelset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                   ctypes.c_double, ctypes.POINTER(LevelSetSpec)]),
+    "casl_forest_build_nodes": (ctypes.c_int, [ctypes.c_void_p]),
+    "casl_forest_sizes": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p, c_int32_p]),
+    "casl_forest_get_leaves": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int8_p, c_int32_p]),
+    "casl_forest_get_nodes": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_double_p]),
+    "casl_forest_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LevelSetSpec), c_double_p]),
+    "casl_forest_init_neighbors": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FlatStencil)]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads the native library once; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(code, func):
+    if code != CASL_OK:
+        msg = lib().casl_last_error()
+        raise CaslError(code, func, msg.decode() if msg else "")
+    return code
+
+
+def call(name, *args):
+    """Calls a C-ABI function and raises CaslError on a non-zero status."""
+    return check(getattr(lib(), name)(*args), name)

@@ -1,0 +1,427 @@
+// Batched reinitialization of many independent small trees in one
+// persistent kernel (reinitialize_2nd_order, src/my_p4est_level_set.h:468,
+// applied to every tree).
+//
+// The trees' stencils are concatenated into one table set (node indices
+// offset per tree).  A persistent grid of workgroups pulls trees from a
+// device work queue, largest first, and runs the whole call for its tree --
+// phi_0 copy, pass A on phi_0, the phi_0-only prep, then every pseudo-time
+// stage -- separated by workgroup barriers only: stencils never cross
+// trees, so no grid-wide synchronisation is needed.  The per-node arithmetic
+// is the same device code as the grid-wide kernels, so each tree's result is
+// bit-identical to casl_reinit on that tree alone.
+//
+// Launch shape (threads per block x nodes per thread) is picked by
+// casl_batch_autotune; casl_batch_reinit_graph records the queue reset and
+// the persistent kernel into a hipGraph once and replays it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <vector>
+
+#include "reinit_device.hpp"
+
+using namespace casl;
+using namespace casl::dev;
+
+namespace {
+
+template <int D, class R>
+struct BatchArgs {
+  Tables<D> t;
+  const int64_t *__restrict__ off;    // [n_trees + 1]
+  const int32_t *__restrict__ order;  // processing order, largest tree first
+  int32_t n_trees;
+  unsigned int *__restrict__ queue;   // work counter
+  double *__restrict__ phi;           // in/out, FP64
+  double *__restrict__ phi0;
+  double *__restrict__ dd0;
+  uint16_t *__restrict__ flags;
+  R *__restrict__ sfix;
+  R *__restrict__ cur;
+  R *__restrict__ nxt;
+  R *__restrict__ ddn;
+  int n_iter;
+  double zdt;
+};
+
+// Runs body(n) for every node n of [b, e): each thread takes NPT nodes a
+// blockDim apart per sweep so their gathers are in flight together.
+template <int BT, int NPT, class Body>
+__device__ __forceinline__ void for_nodes(int64_t b, int64_t e, Body body) {
+  for (int64_t base = b + threadIdx.x; base < e; base += int64_t(BT) * NPT) {
+#pragma unroll
+    for (int j = 0; j < NPT; ++j) {
+      const int64_t n = base + int64_t(j) * BT;
+      if (n < e) body(n);
+    }
+  }
+}
+
+template <int D, class R, int BT, int NPT>
+__global__ void __launch_bounds__(BT) k_batch(BatchArgs<D, R> A) {
+  __shared__ int32_t s_tree;
+  const Tables<D> &t = A.t;
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const unsigned k = atomicAdd(A.queue, 1u);
+      s_tree = (int32_t(k) < A.n_trees) ? A.order[k] : -1;
+    }
+    __syncthreads();
+    const int32_t tree = s_tree;
+    __syncthreads();  // every thread has read s_tree before thread 0 rewrites it
+    if (tree < 0) break;
+    const int64_t b = A.off[tree], e = A.off[tree + 1];
+    for_nodes<BT, NPT>(b, e, [&](int64_t n) {
+      const double v = A.phi[n];
+      A.phi0[n] = v;
+      A.cur[n] = R(v);
+    });
+    __syncthreads();
+    for_nodes<BT, NPT>(b, e, [&](int64_t n) { node_dd<D, double>(t, n, A.phi0, A.dd0); });
+    __syncthreads();
+    for_nodes<BT, NPT>(b, e, [&](int64_t n) {
+      node_prep<D, R>(t, n, A.phi0, A.dd0, nullptr, -DBL_MAX, DBL_MAX, A.zdt, true, A.flags, A.sfix);
+    });
+    __syncthreads();
+    for (int it = 0; it < A.n_iter; ++it) {
+      for_nodes<BT, NPT>(b, e, [&](int64_t n) { node_dd<D, R>(t, n, A.cur, A.ddn); });
+      __syncthreads();
+      for_nodes<BT, NPT>(b, e, [&](int64_t n) {
+        A.nxt[n] = node_stage<D, R, true>(t, n, A.flags[n], A.sfix, A.phi0, A.cur, A.ddn);
+      });
+      __syncthreads();
+      for_nodes<BT, NPT>(b, e, [&](int64_t n) { node_dd<D, R>(t, n, A.nxt, A.ddn); });
+      __syncthreads();
+      for_nodes<BT, NPT>(b, e, [&](int64_t n) {
+        const R r = node_stage<D, R, true>(t, n, A.flags[n], A.sfix, A.phi0, A.nxt, A.ddn);
+        A.cur[n] = R(0.5) * (A.cur[n] + r);
+      });
+      __syncthreads();
+    }
+    for_nodes<BT, NPT>(b, e, [&](int64_t n) { A.phi[n] = double(A.cur[n]); });
+    __syncthreads();
+  }
+}
+
+__global__ void k_reset_queue(unsigned int *q) { *q = 0u; }
+
+struct Schedule {
+  int block = 256;
+  int npt = 1;
+};
+
+const int kBlocks[] = {64, 128, 256, 512};
+const int kNpts[] = {1, 2};
+
+template <int D, class R>
+using KernelFn = void (*)(BatchArgs<D, R>);
+
+template <int D, class R>
+KernelFn<D, R> pick_kernel(const Schedule &s) {
+#define CASL_PICK(BT, NPT) \
+  if (s.block == BT && s.npt == NPT) return &k_batch<D, R, BT, NPT>;
+  CASL_PICK(64, 1) CASL_PICK(64, 2) CASL_PICK(128, 1) CASL_PICK(128, 2) CASL_PICK(256, 1) CASL_PICK(256, 2)
+  CASL_PICK(512, 1) CASL_PICK(512, 2)
+#undef CASL_PICK
+  throw std::invalid_argument("unsupported batch schedule (block in {64,128,256,512}, nodes/thread in {1,2})");
+}
+
+}  // namespace
+
+struct casl_batch {
+  casl_stencil st;  // concatenated tables + scratch (phi0, dd0, flags)
+  int32_t n_trees = 0;
+  int32_t max_tree = 0;
+  casl::DevBuf<int64_t> off;
+  casl::DevBuf<int32_t> order;
+  casl::DevBuf<unsigned int> queue;
+  Schedule sched;
+  int cus = 0;
+  // graph cache
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double *g_phi = nullptr;
+  int g_iter = -1, g_prec = -1, g_block = 0, g_npt = 0;
+
+  ~casl_batch() { drop_graph(); }
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
+};
+
+namespace {
+
+template <int D, class R>
+void launch_batch(casl_batch *B, double *phi, int n_iter, hipStream_t s, bool allocate_only) {
+  casl_stencil &st = B->st;
+  const int64_t N = st.N;
+  constexpr int F = 2 * D;
+  st.phi0.ensure(N);
+  st.dd0.ensure(size_t(D) * N);
+  st.flags.ensure(N);
+  BatchArgs<D, R> A;
+  A.t = st.tables<D>();
+  A.off = B->off.p;
+  A.order = B->order.p;
+  A.n_trees = B->n_trees;
+  A.queue = B->queue.p;
+  A.phi = phi;
+  A.phi0 = st.phi0.p;
+  A.dd0 = st.dd0.p;
+  A.flags = st.flags.p;
+  A.n_iter = n_iter;
+  A.zdt = st.zdt;
+  // scratch: arms, phi_np1 and phi_n working copies, pass-A output
+  if constexpr (sizeof(R) == 8) {
+    st.sfix64.ensure(size_t(F) * N);
+    st.tmp64.ensure(2 * size_t(N));
+    st.ddn64.ensure(size_t(D) * N);
+    A.sfix = st.sfix64.p;
+    A.nxt = st.tmp64.p;
+    A.cur = st.tmp64.p + N;
+    A.ddn = st.ddn64.p;
+  } else {
+    st.sfixf.ensure(size_t(F) * N);
+    st.tmpf.ensure(2 * size_t(N));
+    st.ddnf.ensure(size_t(D) * N);
+    A.sfix = st.sfixf.p;
+    A.nxt = st.tmpf.p;
+    A.cur = st.tmpf.p + N;
+    A.ddn = st.ddnf.p;
+  }
+  if (allocate_only) return;
+  KernelFn<D, R> fn = pick_kernel<D, R>(B->sched);
+  int occ = 0;
+  HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, B->sched.block, 0));
+  occ = std::max(occ, 1);
+  const int grid = std::max(1, std::min<int>(B->n_trees, B->cus * occ));
+  hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, B->queue.p);
+  prof_begin("k_batch", s);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(B->sched.block), 0, s, A);
+  prof_end("k_batch", s);
+  HIP_CHECK(hipGetLastError());
+}
+
+void batch_run(casl_batch *B, double *phi, int n_iter, int precision, hipStream_t s, bool allocate_only = false) {
+  if (B->st.dim == 2) {
+    if (precision == CASL_PRECISION_FP64)
+      launch_batch<2, double>(B, phi, n_iter, s, allocate_only);
+    else
+      launch_batch<2, float>(B, phi, n_iter, s, allocate_only);
+  } else {
+    if (precision == CASL_PRECISION_FP64)
+      launch_batch<3, double>(B, phi, n_iter, s, allocate_only);
+    else
+      launch_batch<3, float>(B, phi, n_iter, s, allocate_only);
+  }
+}
+
+void check_run_args(casl_batch *B, double *phi, int n_iter, int precision) {
+  if (!B || !phi) throw std::invalid_argument("null argument");
+  if (n_iter < 0) throw std::invalid_argument("negative number of iterations");
+  if (precision != CASL_PRECISION_FP64 && precision != CASL_PRECISION_MIXED)
+    throw std::invalid_argument("unknown precision mode");
+}
+
+}  // namespace
+
+extern "C" {
+
+int casl_batch_create(const casl_flat_stencil *trees, int32_t n_trees, casl_batch_t **out) {
+  return guarded([&]() {
+    if (!out || !trees || n_trees <= 0) throw std::invalid_argument("bad arguments");
+    *out = nullptr;
+    const int D = trees[0].dim;
+    if (D != 2 && D != 3) throw std::invalid_argument("dim must be 2 or 3");
+    const int F = 2 * D, C = 1 << (D - 1), T = 2 * (D - 1);
+    std::vector<int64_t> off(size_t(n_trees) + 1, 0);
+    double zdt = trees[0].zero_distance_threshold;
+    for (int32_t k = 0; k < n_trees; ++k) {
+      if (trees[k].dim != D) throw std::invalid_argument("all trees must have the same dimension");
+      if (trees[k].n_nodes <= 0) throw std::invalid_argument("empty tree");
+      if (trees[k].zero_distance_threshold != zdt)
+        throw std::invalid_argument("all trees must share the zero distance threshold");
+      off[k + 1] = off[k] + trees[k].n_nodes;
+    }
+    const int64_t N = off[n_trees];
+    if (N >= int64_t(INT32_MAX)) throw std::invalid_argument("batch too large for int32 node indices");
+    std::vector<int32_t> nbr(size_t(F) * C * N);
+    std::vector<double> d(size_t(F) * N), dtr(size_t(F) * T * N);
+    std::vector<uint8_t> wall(N);
+    int32_t max_tree = 0;
+    for (int32_t k = 0; k < n_trees; ++k) {
+      const casl_flat_stencil &tr = trees[k];
+      const int64_t n = tr.n_nodes, o = off[k];
+      max_tree = std::max<int32_t>(max_tree, int32_t(n));
+      for (int fc = 0; fc < F * C; ++fc)
+        for (int64_t i = 0; i < n; ++i) {
+          const int32_t v = tr.nbr[size_t(fc) * n + i];
+          if (v < 0 || v >= n) throw std::runtime_error("tree stencil index out of range");
+          nbr[size_t(fc) * N + o + i] = int32_t(o + v);
+        }
+      for (int f = 0; f < F; ++f) std::memcpy(&d[size_t(f) * N + o], tr.d + size_t(f) * n, n * sizeof(double));
+      for (int ft = 0; ft < F * T; ++ft)
+        std::memcpy(&dtr[size_t(ft) * N + o], tr.d_tr + size_t(ft) * n, n * sizeof(double));
+      std::memcpy(&wall[o], tr.wall, n);
+    }
+    auto B = std::make_unique<casl_batch>();
+    casl_flat_stencil all{};
+    all.dim = D;
+    all.n_nodes = N;
+    all.nbr = nbr.data();
+    all.d = d.data();
+    all.d_tr = dtr.data();
+    all.wall = wall.data();
+    all.zero_distance_threshold = zdt;
+    upload_tables(&all, &B->st);
+    B->n_trees = n_trees;
+    B->max_tree = max_tree;
+    std::vector<int32_t> order(n_trees);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+    B->off.ensure(off.size());
+    B->order.ensure(order.size());
+    B->queue.ensure(1);
+    HIP_CHECK(hipMemcpy(B->off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(B->order.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, B->st.device));
+    B->cus = prop.multiProcessorCount;
+    *out = B.release();
+  });
+}
+
+int casl_batch_destroy(casl_batch_t *b) {
+  delete b;
+  return CASL_OK;
+}
+
+int casl_batch_info(const casl_batch_t *b, int64_t *total_nodes, int32_t *n_trees, int32_t *max_tree_nodes) {
+  return guarded([&]() {
+    if (!b) throw std::invalid_argument("null batch");
+    if (total_nodes) *total_nodes = b->st.N;
+    if (n_trees) *n_trees = b->n_trees;
+    if (max_tree_nodes) *max_tree_nodes = b->max_tree;
+  });
+}
+
+int casl_batch_set_schedule(casl_batch_t *b, int32_t block_threads, int32_t nodes_per_thread) {
+  return guarded([&]() {
+    if (!b) throw std::invalid_argument("null batch");
+    Schedule s;
+    s.block = block_threads > 0 ? block_threads : 256;
+    s.npt = nodes_per_thread > 0 ? nodes_per_thread : 1;
+    (void)pick_kernel<2, double>(s);  // validates the pair
+    b->sched = s;
+  });
+}
+
+int casl_batch_get_schedule(const casl_batch_t *b, int32_t *block_threads, int32_t *nodes_per_thread) {
+  return guarded([&]() {
+    if (!b) throw std::invalid_argument("null batch");
+    if (block_threads) *block_threads = b->sched.block;
+    if (nodes_per_thread) *nodes_per_thread = b->sched.npt;
+  });
+}
+
+int casl_batch_reinit(casl_batch_t *b, double *phi, int n_iter, int precision, void *stream) {
+  return guarded([&]() {
+    check_run_args(b, phi, n_iter, precision);
+    batch_run(b, phi, n_iter, precision, static_cast<hipStream_t>(stream));
+  });
+}
+
+int casl_batch_reinit_graph(casl_batch_t *b, double *phi, int n_iter, int precision, void *stream) {
+  return guarded([&]() {
+    check_run_args(b, phi, n_iter, precision);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!b->exec || b->g_phi != phi || b->g_iter != n_iter || b->g_prec != precision ||
+        b->g_block != b->sched.block || b->g_npt != b->sched.npt) {
+      b->drop_graph();
+      // scratch must exist before capture (no allocation inside the graph)
+      hipStream_t cap;
+      HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      try {
+        batch_run(b, phi, n_iter, precision, cap, true);  // scratch only, nothing launched
+        HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        batch_run(b, phi, n_iter, precision, cap);
+        HIP_CHECK(hipStreamEndCapture(cap, &b->graph));
+        HIP_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+      } catch (...) {
+        (void)hipStreamDestroy(cap);
+        b->drop_graph();
+        throw;
+      }
+      HIP_CHECK(hipStreamDestroy(cap));
+      b->g_phi = phi;
+      b->g_iter = n_iter;
+      b->g_prec = precision;
+      b->g_block = b->sched.block;
+      b->g_npt = b->sched.npt;
+    }
+    HIP_CHECK(hipGraphLaunch(b->exec, s));
+  });
+}
+
+int casl_batch_autotune(casl_batch_t *b, const double *phi, int n_iter, int precision, void *stream,
+                        int32_t *best_block, int32_t *best_npt, float *best_ms) {
+  return guarded([&]() {
+    check_run_args(b, const_cast<double *>(phi), n_iter, precision);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    const int64_t N = b->st.N;
+    double *work = nullptr;
+    HIP_CHECK(hipMalloc(&work, sizeof(double) * N));
+    hipEvent_t e0, e1;
+    HIP_CHECK(hipEventCreate(&e0));
+    HIP_CHECK(hipEventCreate(&e1));
+    Schedule best = b->sched;
+    float best_t = 1e30f;
+    try {
+      for (int bt : kBlocks)
+        for (int npt : kNpts) {
+          b->sched.block = bt;
+          b->sched.npt = npt;
+          float tsum = 0;
+          for (int rep = 0; rep < 3; ++rep) {
+            HIP_CHECK(hipMemcpyAsync(work, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+            HIP_CHECK(hipEventRecord(e0, s));
+            batch_run(b, work, n_iter, precision, s);
+            HIP_CHECK(hipEventRecord(e1, s));
+            HIP_CHECK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0) tsum += ms;  // first run warms caches
+          }
+          const float t = tsum / 2;
+          if (t < best_t) {
+            best_t = t;
+            best.block = bt;
+            best.npt = npt;
+          }
+        }
+    } catch (...) {
+      (void)hipFree(work);
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      b->sched = best;
+      throw;
+    }
+    (void)hipFree(work);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    b->sched = best;
+    if (best_block) *best_block = best.block;
+    if (best_npt) *best_npt = best.npt;
+    if (best_ms) *best_ms = best_t;
+  });
+}
+
+}  // extern "C"

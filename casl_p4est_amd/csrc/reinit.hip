@@ -1,0 +1,428 @@
+// Level-set reinitialization on MI355X (gfx950): grid-wide kernels and the
+// stencil part of the C ABI of include/casl_reinit.h.
+//
+// Per reinitialize_within_range_of_phi_0 call (src/my_p4est_level_set.cpp:290-557):
+//   1. phi_0 = phi (device copy);  dd0 = second_derivatives_central(phi_0)
+//   2. k_prep: everything that depends on phi_0 and the grid only -- the
+//      anchoring / band / mask mode, the sign of phi_0 and, for nodes next to
+//      the interface, the subcell arm lengths s (interface_Location*), which
+//      the reference recomputes identically at every stage (:86-211).  FP64.
+//   3. per pseudo-time stage: k_dd (pass A, second_derivatives_central of the
+//      stage input) then k_stage (pass B, reinitialize_one_iteration :59-288);
+//      the RK2 average phi_n = (phi_n + phi_np2)/2 (:497-498) is fused into
+//      the second stage's store.
+// The mixed path runs step 3 in FP32 (values, second derivatives, Godunov
+// update) and keeps step 2 -- the subcell root finding where phi values
+// cancel -- and the band error reduction in FP64.
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
+#include "reinit_device.hpp"
+
+using namespace casl;
+using namespace casl::dev;
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+inline unsigned grid_for(int64_t n, int block = BLOCK) { return unsigned((n + block - 1) / block); }
+
+template <int D, class R>
+__global__ void __launch_bounds__(BLOCK) k_dd(Tables<D> t, const R *__restrict__ f, R *__restrict__ fdd) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= t.N) return;
+  node_dd<D, R>(t, n, f, fdd);
+}
+
+template <int D, class S>
+__global__ void __launch_bounds__(BLOCK)
+    k_prep(Tables<D> t, const double *__restrict__ phi0, const double *__restrict__ dd0,
+           const uint8_t *__restrict__ mask, double lim_low, double lim_high, double zdt, int second_order,
+           uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= t.N) return;
+  node_prep<D, S>(t, n, phi0, dd0, mask, lim_low, lim_high, zdt, second_order != 0, flags, s_fix);
+}
+
+// Pass B of one pseudo-time stage.  out[n] = result, or for the second RK2
+// stage out[n] = 0.5*(out[n] + result): out then holds phi_n, each node reads
+// and writes only its own entry, and phin is a different buffer.
+template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
+__global__ void __launch_bounds__(BLOCK)
+    k_stage(Tables<D> t, const uint16_t *__restrict__ flags, const R *__restrict__ s_fix,
+            const double *__restrict__ phi0, const R *__restrict__ phin, const R *__restrict__ dd, R *out) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= t.N) return;
+  const R r = node_stage<D, R, SECOND_ORDER>(t, n, flags[n], s_fix, phi0, phin, dd);
+  if constexpr (RK_AVG)
+    out[n] = R(0.5) * (out[n] + r);
+  else
+    out[n] = r;
+}
+
+__global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = float(a[i]);
+}
+__global__ void k_f2d(const float *__restrict__ a, double *__restrict__ b, int64_t n) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = double(a[i]);
+}
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+// max |a - b| over nodes with |ref| < band; non-negative doubles order like
+// their bit patterns, so an unsigned 64-bit atomicMax reduces them.
+__global__ void __launch_bounds__(BLOCK) k_band_linf(const double *__restrict__ a, const double *__restrict__ b,
+                                                     const double *__restrict__ ref, double band, int64_t n,
+                                                     unsigned long long *__restrict__ result) {
+  __shared__ double red[BLOCK];
+  double m = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+    if (fabs(ref[i]) < band) m = fmax(m, fabs(a[i] - b[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = BLOCK / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMax(result, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+void require_device() {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0) throw hip_error("no HIP device available");
+}
+
+// reinitialize_within_range_of_phi_0 driver, templated on dimension and the
+// stage arithmetic type R (double: FP64, float: mixed).
+template <int D, class R>
+void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_space, int order_time, int n_iter,
+                double limit_high, double limit_low, hipStream_t s) {
+  const int64_t N = st->N;
+  const Tables<D> t = st->tables<D>();
+  double lim_low, lim_high;
+  widened_limits(limit_high, limit_low, st->zdt, lim_low, lim_high);
+  const bool so = order_space > 1;
+  constexpr int F = 2 * D;
+  const unsigned grid = grid_for(N);
+
+  st->phi0.ensure(N);
+  st->flags.ensure(N);
+  if (so) st->dd0.ensure(size_t(D) * N);
+  HIP_CHECK(hipMemcpyAsync(st->phi0.p, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  if (so) {
+    prof_begin("k_dd", s);
+    hipLaunchKernelGGL((k_dd<D, double>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p);
+    prof_end("k_dd", s);
+    HIP_CHECK(hipGetLastError());
+  }
+  R *sfix, *cur, *nxt, *ddn;
+  if constexpr (sizeof(R) == 8) {
+    st->sfix64.ensure(size_t(F) * N);
+    st->tmp64.ensure(N);
+    if (so) st->ddn64.ensure(size_t(D) * N);
+    sfix = st->sfix64.p;
+    cur = phi;
+    nxt = st->tmp64.p;
+    ddn = st->ddn64.p;
+  } else {
+    st->sfixf.ensure(size_t(F) * N);
+    st->phif.ensure(N);
+    st->tmpf.ensure(N);
+    if (so) st->ddnf.ensure(size_t(D) * N);
+    sfix = st->sfixf.p;
+    cur = st->phif.p;
+    nxt = st->tmpf.p;
+    ddn = st->ddnf.p;
+    hipLaunchKernelGGL(k_d2f, dim3(grid), dim3(BLOCK), 0, s, phi, st->phif.p, N);
+    HIP_CHECK(hipGetLastError());
+  }
+  prof_begin("k_prep", s);
+  hipLaunchKernelGGL((k_prep<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p, mask, lim_low,
+                     lim_high, st->zdt, int(so), st->flags.p, sfix);
+  prof_end("k_prep", s);
+  HIP_CHECK(hipGetLastError());
+
+  auto dd_pass = [&](const R *in) {
+    prof_begin("k_dd", s);
+    hipLaunchKernelGGL((k_dd<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, in, ddn);
+    prof_end("k_dd", s);
+  };
+  auto stage_launch = [&](const R *in, R *out, bool avg) {
+    if (so) {
+      if (avg)
+        hipLaunchKernelGGL((k_stage<D, R, true, true>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
+                           st->phi0.p, in, ddn, out);
+      else
+        hipLaunchKernelGGL((k_stage<D, R, true, false>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
+                           st->phi0.p, in, ddn, out);
+    } else {
+      if (avg)
+        hipLaunchKernelGGL((k_stage<D, R, false, true>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
+                           st->phi0.p, in, ddn, out);
+      else
+        hipLaunchKernelGGL((k_stage<D, R, false, false>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
+                           st->phi0.p, in, ddn, out);
+    }
+  };
+  auto stage = [&](const R *in, R *out, bool avg) {
+    prof_begin("k_stage", s);
+    stage_launch(in, out, avg);
+    prof_end("k_stage", s);
+  };
+  for (int it = 0; it < n_iter; ++it) {
+    if (so) dd_pass(cur);
+    stage(cur, nxt, false);  // phi_n -> phi_np1
+    if (order_time > 1) {
+      if (so) dd_pass(nxt);
+      stage(nxt, cur, true);  // phi_n <- (phi_n + stage(phi_np1)) / 2, in place
+    } else {
+      std::swap(cur, nxt);  // memcpy(phi_n, phi_np1) (:521)
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+  if constexpr (sizeof(R) == 8) {
+    if (cur != phi) HIP_CHECK(hipMemcpyAsync(phi, cur, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  } else {
+    hipLaunchKernelGGL(k_f2d, dim3(grid), dim3(BLOCK), 0, s, cur, phi, N);
+    HIP_CHECK(hipGetLastError());
+  }
+}
+
+void validate_tables(const casl_flat_stencil *tab) {
+  if (!tab) throw std::invalid_argument("null stencil tables");
+  if (tab->dim != 2 && tab->dim != 3) throw std::invalid_argument("dim must be 2 or 3");
+  if (tab->n_nodes <= 0 || tab->n_nodes >= int64_t(INT32_MAX)) throw std::invalid_argument("bad n_nodes");
+  if (!tab->nbr || !tab->d || !tab->d_tr || !tab->wall) throw std::invalid_argument("null table pointer");
+  if (!(tab->zero_distance_threshold > 0)) throw std::invalid_argument("zero_distance_threshold must be > 0");
+}
+
+}  // namespace
+
+namespace casl {
+
+void upload_tables(const casl_flat_stencil *tab, casl_stencil *st) {
+  validate_tables(tab);
+  require_device();
+  const int D = tab->dim;
+  const int64_t N = tab->n_nodes;
+  const int F = 2 * D, C = 1 << (D - 1), T = 2 * (D - 1);
+  const size_t nidx = size_t(F) * C * N;
+  for (size_t i = 0; i < nidx; ++i)
+    if (tab->nbr[i] < 0 || tab->nbr[i] >= N)
+      throw std::runtime_error("stencil node index out of range at entry " + std::to_string(i));
+  const size_t nd = size_t(F) * N, nt = size_t(F) * T * N;
+  std::vector<float> dist(nd + nt);
+  for (size_t i = 0; i < nd; ++i) {
+    dist[i] = float(tab->d[i]);
+    if (double(dist[i]) != tab->d[i]) throw std::domain_error("face distance not exact in FP32");
+    if (!(tab->d[i] != 0.0)) throw std::runtime_error("zero face distance");
+  }
+  for (size_t i = 0; i < nt; ++i) {
+    dist[nd + i] = float(tab->d_tr[i]);
+    if (double(dist[nd + i]) != tab->d_tr[i]) throw std::domain_error("transverse distance not exact in FP32");
+  }
+  st->dim = D;
+  st->N = N;
+  st->zdt = tab->zero_distance_threshold;
+  HIP_CHECK(hipGetDevice(&st->device));
+  st->nbr.ensure(nidx);
+  st->dist.ensure(dist.size());
+  st->wall.ensure(N);
+  HIP_CHECK(hipMemcpy(st->nbr.p, tab->nbr, nidx * sizeof(int32_t), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(st->dist.p, dist.data(), dist.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_CHECK(hipMemcpy(st->wall.p, tab->wall, N * sizeof(uint8_t), hipMemcpyHostToDevice));
+}
+
+// ------------------------------------------------------------ kernel timing
+namespace {
+struct Profiler {
+  std::mutex mu;
+  std::atomic<bool> on{false};
+  struct Pending {
+    hipEvent_t a = nullptr, b = nullptr;
+  };
+  std::map<std::string, std::vector<Pending>> events;
+  std::map<std::string, Pending> open;
+  std::map<std::string, std::pair<double, int64_t>> done;
+
+  void flush(const std::string &k) {
+    auto &v = events[k];
+    for (auto &p : v) {
+      float ms = 0.f;
+      if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+        done[k].first += ms;
+        done[k].second += 1;
+      }
+      (void)hipEventDestroy(p.a);
+      (void)hipEventDestroy(p.b);
+    }
+    v.clear();
+  }
+};
+Profiler &profiler() {
+  static Profiler p;
+  return p;
+}
+}  // namespace
+
+bool prof_on() { return profiler().on.load(std::memory_order_relaxed); }
+
+void prof_begin(const char *kernel, hipStream_t s) {
+  Profiler &P = profiler();
+  if (!P.on) return;
+  std::lock_guard<std::mutex> g(P.mu);
+  Profiler::Pending p;
+  HIP_CHECK(hipEventCreate(&p.a));
+  HIP_CHECK(hipEventCreate(&p.b));
+  HIP_CHECK(hipEventRecord(p.a, s));
+  P.open[kernel] = p;
+}
+
+void prof_end(const char *kernel, hipStream_t s) {
+  Profiler &P = profiler();
+  if (!P.on) return;
+  std::lock_guard<std::mutex> g(P.mu);
+  auto it = P.open.find(kernel);
+  if (it == P.open.end()) return;
+  HIP_CHECK(hipEventRecord(it->second.b, s));
+  auto &v = P.events[kernel];
+  v.push_back(it->second);
+  P.open.erase(it);
+  if (v.size() >= 4096) P.flush(kernel);
+}
+
+}  // namespace casl
+
+extern "C" {
+
+int casl_profile_enable(int enable) {
+  casl::profiler().on = (enable != 0);
+  return CASL_OK;
+}
+
+int casl_profile_reset(void) {
+  return guarded([&]() {
+    Profiler &P = casl::profiler();
+    std::lock_guard<std::mutex> g(P.mu);
+    for (auto &kv : P.events) P.flush(kv.first);
+    P.done.clear();
+  });
+}
+
+int casl_profile_query(const char *kernel, double *total_ms, int64_t *launches) {
+  return guarded([&]() {
+    if (!kernel) throw std::invalid_argument("null kernel name");
+    Profiler &P = casl::profiler();
+    std::lock_guard<std::mutex> g(P.mu);
+    P.flush(kernel);
+    auto it = P.done.find(kernel);
+    if (total_ms) *total_ms = it == P.done.end() ? 0.0 : it->second.first;
+    if (launches) *launches = it == P.done.end() ? 0 : it->second.second;
+  });
+}
+
+int casl_stencil_create(const casl_flat_stencil *tab, casl_stencil_t **out) {
+  return guarded([&]() {
+    if (!out) throw std::invalid_argument("null out");
+    *out = nullptr;
+    auto st = std::make_unique<casl_stencil>();
+    upload_tables(tab, st.get());
+    *out = st.release();
+  });
+}
+
+int casl_stencil_destroy(casl_stencil_t *st) {
+  delete st;
+  return CASL_OK;
+}
+
+int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes) {
+  return guarded([&]() {
+    if (!st) throw std::invalid_argument("null stencil");
+    if (dim) *dim = st->dim;
+    if (n_nodes) *n_nodes = st->N;
+  });
+}
+
+int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd, void *stream) {
+  return guarded([&]() {
+    if (!st || !f || !fdd) throw std::invalid_argument("null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (st->dim == 2)
+      hipLaunchKernelGGL((k_dd<2, double>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<2>(), f, fdd);
+    else
+      hipLaunchKernelGGL((k_dd<3, double>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<3>(), f, fdd);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+static int reinit_common(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space, int order_time,
+                         int n_iter, double limit_high, double limit_low, int precision, void *stream) {
+  return guarded([&]() {
+    if (!st || !phi) throw std::invalid_argument("null argument");
+    if (order_space == 0 || order_space > 2 || order_time == 0 || order_time > 2)
+      throw std::invalid_argument(
+          "reinitialize_within_range_of_phi_0: valid orders of accuracy for space and pseudotime are 1 and 2 only!");
+    if (n_iter < 0) throw std::invalid_argument("negative number of iterations");
+    if (!(limit_high > limit_low)) throw std::invalid_argument("limit_high must exceed limit_low");
+    if (precision != CASL_PRECISION_FP64 && precision != CASL_PRECISION_MIXED)
+      throw std::invalid_argument("unknown precision mode");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (st->dim == 2) {
+      if (precision == CASL_PRECISION_FP64)
+        run_reinit<2, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+      else
+        run_reinit<2, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+    } else {
+      if (precision == CASL_PRECISION_FP64)
+        run_reinit<3, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+      else
+        run_reinit<3, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+    }
+  });
+}
+
+int casl_reinit(casl_stencil_t *st, double *phi, int order_space, int order_pseudotime, int n_iter, double limit_high,
+                double limit_low, int precision, void *stream) {
+  return reinit_common(st, phi, nullptr, order_space, order_pseudotime, n_iter, limit_high, limit_low, precision,
+                       stream);
+}
+
+int casl_reinit_masked(casl_stencil_t *st, double *phi, const uint8_t *mask, int n_iter, double limit_high,
+                       double limit_low, int precision, void *stream) {
+  if (!mask) {
+    set_error("null mask");
+    return CASL_ERR_INVALID_ARGUMENT;
+  }
+  return reinit_common(st, phi, mask, 2, 2, n_iter, limit_high, limit_low, precision, stream);
+}
+
+int casl_band_linf(casl_stencil_t *st, const double *a, const double *b, const double *ref, double band,
+                   double *result, void *stream) {
+  return guarded([&]() {
+    if (!st || !a || !b || !ref || !result) throw std::invalid_argument("null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    st->red.ensure(1);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, st->red.p);
+    const unsigned grid = std::min<unsigned>(grid_for(st->N), 2048);
+    hipLaunchKernelGGL(k_band_linf, dim3(grid), dim3(BLOCK), 0, s, a, b, ref, band, st->N, st->red.p);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    HIP_CHECK(hipMemcpyAsync(&bits, st->red.p, sizeof(bits), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    double v;
+    std::memcpy(&v, &bits, sizeof(v));
+    *result = v;
+  });
+}
+
+}  // extern "C"

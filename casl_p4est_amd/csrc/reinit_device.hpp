@@ -1,0 +1,291 @@
+// Per-node bodies of the reinitialization passes, shared by the grid-wide
+// kernels (reinit.hip) and the persistent batched kernel (batch.hip), plus
+// the host-side device-buffer helpers.  Paths are relative to the reference
+// tree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cmath>
+#include <new>
+#include <stdexcept>
+#include <string>
+
+#include "../../include/casl_reinit.h"
+#include "stencil_math.hpp"
+
+namespace casl {
+void set_error(const std::string &msg);
+
+struct hip_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIP_CHECK(expr)                                                                         \
+  do {                                                                                          \
+    hipError_t _e = (expr);                                                                     \
+    if (_e != hipSuccess)                                                                       \
+      throw ::casl::hip_error(std::string(#expr) + " failed: " + hipGetErrorString(_e));        \
+  } while (0)
+
+template <class T>
+struct DevBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (n >= count && p) return;
+    release();
+    if (count == 0) return;
+    hipError_t e = hipMalloc(&p, count * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+      throw hip_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    n = count;
+  }
+};
+
+// Maps exceptions to the C ABI status codes.
+template <class Fn>
+int guarded(Fn fn) {
+  try {
+    fn();
+    return CASL_OK;
+  } catch (const std::invalid_argument &e) {
+    set_error(e.what());
+    return CASL_ERR_INVALID_ARGUMENT;
+  } catch (const std::bad_alloc &) {
+    set_error("device or host allocation failed");
+    return CASL_ERR_OUT_OF_MEMORY;
+  } catch (const hip_error &e) {
+    set_error(e.what());
+    return CASL_ERR_HIP;
+  } catch (const std::domain_error &e) {
+    set_error(e.what());
+    return CASL_ERR_INEXACT_DISTANCES;
+  } catch (const std::exception &e) {
+    set_error(e.what());
+    return CASL_ERR_INVALID_STENCIL;
+  }
+}
+
+inline double largest_dbl_smaller_than_dbl_max() {  // src/my_p4est_utils.h:322-330
+  volatile double p = 1.0;
+  while (DBL_MAX - p == DBL_MAX) p *= 2.0;
+  return DBL_MAX - p;
+}
+
+// Widened phi_0 limits of reinitialize_one_iteration (src/my_p4est_level_set.cpp:75-76).
+inline void widened_limits(double limit_high, double limit_low, double zdt, double &lo, double &hi) {
+  const double ldbl = largest_dbl_smaller_than_dbl_max();
+  lo = (limit_low < -ldbl ? -DBL_MAX : limit_low - dev::max2(zdt, dev::EPS * std::fabs(limit_low)));
+  hi = (limit_high > ldbl ? DBL_MAX : limit_high + dev::max2(zdt, dev::EPS * std::fabs(limit_high)));
+}
+
+namespace dev {
+
+// flags[n] (uint16):
+//   bits 0-1  mode: 0 update, 1 anchor (phi = 0), 2 outside limits (phi = phi_0), 3 masked (phi = phi_n)
+//   bit  2    phi_0 > 0
+//   bits 3-8  subcell crossing on face f (bit 3 + f)
+enum : uint16_t { MODE_UPDATE = 0, MODE_ANCHOR = 1, MODE_COPY0 = 2, MODE_MASKED = 3 };
+
+// Pass A at one node: second_derivatives_central (laplace) of f.
+template <int D, class R>
+__device__ __forceinline__ void node_dd(const Tables<D> &t, int64_t n, const R *__restrict__ f, R *__restrict__ fdd) {
+  R f000, fnb[2 * D], dd[D];
+  laplace<D, R>(t, n, f, f000, fnb, dd);
+#pragma unroll
+  for (int a = 0; a < D; ++a) fdd[int64_t(a) * t.N + n] = dd[a];
+}
+
+// The phi_0-only part of reinitialize_one_iteration (src/my_p4est_level_set.cpp:82-211):
+// mode, sign of phi_0 and the subcell arm lengths of crossing arms,
+// s_fix[f][n] = MAX(interface_Location*(..), zero_distance_threshold).
+// The reference recomputes these identically at every stage.
+template <int D, class S>
+__device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, const double *__restrict__ phi0,
+                                          const double *__restrict__ dd0, const uint8_t *__restrict__ mask,
+                                          double lim_low, double lim_high, double zdt, bool second_order,
+                                          uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
+  const double p = phi0[n];
+  if (mask && mask[n] == 0) {
+    flags[n] = MODE_MASKED;
+    return;
+  }
+  if (fabs(p) < zdt) {
+    flags[n] = MODE_ANCHOR;
+    return;
+  }
+  if (!(lim_low < p && p < lim_high)) {
+    flags[n] = MODE_COPY0;
+    return;
+  }
+  constexpr int F = 2 * D;
+  double p0_000, p0_f[F], dloc[D];
+  ngbd_quadratic<D, double>(t, n, phi0, p0_000, p0_f, dloc);
+  uint16_t fl = MODE_UPDATE | (p0_000 > 0.0 ? 4 : 0);
+  unsigned cross = 0;
+#pragma unroll
+  for (int f = 0; f < F; ++f)
+    if (p0_000 * p0_f[f] < 0.0) cross |= 1u << f;
+  if (cross) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      if (!(cross & (1u << f))) continue;
+      const int a = f >> 1;
+      double s = double(t.d(f, n));
+      if (second_order) {
+        const double dd_000 = dd0[int64_t(a) * t.N + n];
+        const double dd_f = face_linear<D, double>(t, f, n, dd0 + int64_t(a) * t.N);
+        if ((f & 1) == 0)
+          s = -interface_location_2nd(-s, 0.0, p0_f[f], p0_000, dd_f, dd_000);
+        else
+          s = interface_location_2nd(0.0, s, p0_000, p0_f[f], dd_000, dd_f);
+      } else {
+        if ((f & 1) == 0)
+          s = -interface_location(-s, 0.0, p0_f[f], p0_000);
+        else
+          s = interface_location(0.0, s, p0_000, p0_f[f]);
+      }
+      s_fix[int64_t(f) * t.N + n] = S(max2(s, zdt));
+    }
+    fl |= uint16_t(cross << 3);
+  }
+  flags[n] = fl;
+}
+
+// Pass B at one node: the stage result of reinitialize_one_iteration
+// (src/my_p4est_level_set.cpp:82-281) given the prepared flags / s_fix.
+template <int D, class R, bool SECOND_ORDER>
+__device__ __forceinline__ R node_stage(const Tables<D> &t, int64_t n, uint16_t fl, const R *__restrict__ s_fix,
+                                        const double *__restrict__ phi0, const R *__restrict__ phin,
+                                        const R *__restrict__ dd) {
+  constexpr int F = 2 * D;
+  const int mode = fl & 3;
+  if (mode == MODE_ANCHOR) return R(0.0);
+  if (mode == MODE_COPY0) return R(phi0[n]);
+  if (mode == MODE_MASKED) return phin[n];
+  R pn_000, pn_f[F], pn_dd_000[D];
+  // ngbd_with_quadratic_interpolation of phi_n; its laplace is the node's own
+  // pass-A value (same arithmetic), so phi_n_xx_000 comes from here.
+  ngbd_quadratic<D, R>(t, n, phin, pn_000, pn_f, pn_dd_000);
+  R pn_dd_f[F];
+  if constexpr (SECOND_ORDER) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) pn_dd_f[f] = face_linear<D, R>(t, f, n, dd + int64_t(f >> 1) * t.N);
+  }
+  R s[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) s[f] = R(t.d(f, n));
+  const unsigned cross = (fl >> 3) & ((1u << F) - 1);
+  if (cross) {
+#pragma unroll
+    for (int f = 0; f < F; ++f)
+      if (cross & (1u << f)) {
+        s[f] = s_fix[int64_t(f) * t.N + n];
+        pn_f[f] = R(0.0);
+      }
+    // MAX(s, zero_distance_threshold) of the non-crossing arms only changes
+    // negative wall arms, which the Neumann mirror below overwrites.
+  }
+  // Neumann boundary condition on the walls (:216-225)
+  const uint8_t w = t.wall[n];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const int fm = 2 * a, fp = 2 * a + 1;
+    if (w & (1u << fm)) {
+      s[fm] = s[fp];
+      pn_f[fm] = pn_f[fp];
+      if constexpr (SECOND_ORDER) pn_dd_000[a] = pn_dd_f[fm] = pn_dd_f[fp] = R(0.0);
+    }
+    if (w & (1u << fp)) {
+      s[fp] = s[fm];
+      pn_f[fp] = pn_f[fm];
+      if constexpr (SECOND_ORDER) pn_dd_000[a] = pn_dd_f[fm] = pn_dd_f[fp] = R(0.0);
+    }
+  }
+  // one-sided differences with ENO/minmod correction (:230-248)
+  R Dp[D], Dm[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    Dp[a] = (pn_f[2 * a + 1] - pn_000) / s[2 * a + 1];
+    Dm[a] = (pn_000 - pn_f[2 * a]) / s[2 * a];
+  }
+  if constexpr (SECOND_ORDER) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      const R ddm = minmod(pn_dd_f[2 * a], pn_dd_000[a]);
+      Dm[a] += R(0.5) * s[2 * a] * ddm;
+      const R ddp = minmod(pn_dd_f[2 * a + 1], pn_dd_000[a]);
+      Dp[a] -= R(0.5) * s[2 * a + 1] * ddp;
+    }
+  }
+  // local time step and Godunov upwinding (:250-271)
+  const R sgn = (fl & 4) ? R(1.0) : R(-1.0);
+  R dt = min2(s[0], s[1]);
+  dt = min3(dt, s[2], s[3]);
+  if constexpr (D == 3) dt = min3(dt, s[4], s[5]);
+  dt /= R(D);
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    if (sgn * Dp[a] > R(0.0)) Dp[a] = R(0.0);
+    if (sgn * Dm[a] < R(0.0)) Dm[a] = R(0.0);
+  }
+  R sum = max2(Dp[0] * Dp[0], Dm[0] * Dm[0]) + max2(Dp[1] * Dp[1], Dm[1] * Dm[1]);
+  if constexpr (D == 3) sum = sum + max2(Dp[2] * Dp[2], Dm[2] * Dm[2]);
+  R r = pn_000 - dt * sgn * (sqrt(sum) - R(1.));
+  // sign guard phi_0*phi_np1 < 0 -> flip (:273-274); phi_0 != 0 for updated nodes
+  if ((fl & 4) ? (r < R(0.0)) : (r > R(0.0))) r *= R(-1.0);
+  return r;
+}
+
+}  // namespace dev
+}  // namespace casl
+
+// Device tables and scratch of one stencil (opaque casl_stencil_t).
+struct casl_stencil {
+  int dim = 0;
+  int64_t N = 0;
+  int device = 0;
+  double zdt = 0;
+  casl::DevBuf<int32_t> nbr;
+  casl::DevBuf<float> dist;
+  casl::DevBuf<uint8_t> wall;
+  // scratch, grown on demand
+  casl::DevBuf<double> phi0, dd0, tmp64, ddn64, sfix64;
+  casl::DevBuf<float> phif, tmpf, ddnf, sfixf;
+  casl::DevBuf<uint16_t> flags;
+  casl::DevBuf<unsigned long long> red;
+
+  template <int D>
+  casl::dev::Tables<D> tables() const {
+    casl::dev::Tables<D> t;
+    t.nbr = nbr.p;
+    t.dist = dist.p;
+    t.wall = wall.p;
+    t.N = N;
+    return t;
+  }
+};
+
+namespace casl {
+// Validates and uploads host tables into st (shared by stencils and batches).
+void upload_tables(const casl_flat_stencil *tab, casl_stencil *st);
+
+// Kernel timing (casl_profile_*): prof_begin/prof_end bracket one launch.
+bool prof_on();
+void prof_begin(const char *kernel, hipStream_t s);
+void prof_end(const char *kernel, hipStream_t s);
+}

@@ -1,0 +1,241 @@
+// Per-node stencil arithmetic shared by the reinitialization kernels.
+//
+// Every function restates one piece of the reference in its own operation
+// order (compile with -ffp-contract=off so FP64 results are bit-identical to
+// a strict IEEE evaluation of the reference expressions).  Paths are
+// relative to the reference tree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace casl {
+namespace dev {
+
+constexpr double EPS = 1e-13;  // src/casl_math.h:23
+
+// Device view of the flattened stencils of casl_flat_stencil.  Distances are
+// stored as FP32 (exact for dyadic trees; checked at upload).
+template <int D>
+struct Tables {
+  static constexpr int F = 2 * D;
+  static constexpr int C = 1 << (D - 1);
+  static constexpr int T = 2 * (D - 1);
+  const int32_t *__restrict__ nbr;  // [F*C][N]
+  const float *__restrict__ dist;   // [F][N] face distances, then [F*T][N] transverse
+  const uint8_t *__restrict__ wall; // [N]
+  int64_t N;
+
+  __device__ __forceinline__ int32_t idx(int f, int c, int64_t n) const { return nbr[(int64_t(f) * C + c) * N + n]; }
+  __device__ __forceinline__ float d(int f, int64_t n) const { return dist[int64_t(f) * N + n]; }
+  __device__ __forceinline__ float dt(int f, int k, int64_t n) const {
+    return dist[(int64_t(F) + int64_t(f) * T + k) * N + n];
+  }
+};
+
+// MINMOD, src/casl_math.h:80-88
+template <class R>
+__device__ __forceinline__ R minmod(R a, R b) {
+  if (a * b <= R(0)) return R(0);
+  if (fabs(a) < fabs(b)) return a;
+  return b;
+}
+template <class R>
+__host__ __device__ __forceinline__ R min2(R a, R b) { return a < b ? a : b; }
+template <class R>
+__host__ __device__ __forceinline__ R min3(R a, R b, R c) { return a < b ? min2(a, c) : min2(b, c); }
+template <class R>
+__host__ __device__ __forceinline__ R max2(R a, R b) { return b < a ? a : b; }
+
+// interface_Location, src/casl_math.cpp:55-59
+__device__ __forceinline__ double interface_location(double a, double b, double fa, double fb) {
+  return 0.5 * (a + b + (a - b) * (fa + fb) / (fb - fa));
+}
+
+// interface_Location_With_Second_Order_Derivative, src/casl_math.cpp:67-97
+__device__ __forceinline__ double interface_location_2nd(double a, double b, double fa, double fb, double fxxa,
+                                                         double fxxb) {
+  if (fabs(fa) < EPS) return a + EPS;
+  if (fabs(fb) < EPS) return b - EPS;
+  double fxx = minmod(fxxa, fxxb);
+  double h = b - a;
+  double c2 = 0.5 * fxx;
+  double c1 = (fb - fa) / h;
+  double c0 = 0.5 * (fb + fa) - h * h / 8 * fxx;
+  double x;
+  if (fabs(c2) < EPS)
+    x = -c0 / c1;
+  else if (fb < 0)
+    x = (-2 * c0) / (c1 - sqrt(c1 * c1 - 4 * c2 * c0));
+  else
+    x = (-2 * c0) / (c1 + sqrt(c1 * c1 - 4 * c2 * c0));
+  return x + 0.5 * (a + b);
+}
+
+// get_linear_interpolation_weights + interpolate_linearly,
+// my_p4est_quad_neighbor_nodes_of_node.h:222-306
+template <int D, class R, class G>
+__device__ __forceinline__ R face_linear(const Tables<D> &t, int f, int64_t n, const G *__restrict__ g) {
+  const R d_m0 = R(t.dt(f, 0, n)), d_p0 = R(t.dt(f, 1, n));
+  if constexpr (D == 2) {
+    const R w_mm = d_p0;
+    const R w_pm = d_m0;
+    const R normalization = (d_m0 + d_p0);
+    return (R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 0, n)]) * w_mm) / normalization;
+  } else {
+    const R d_0m = R(t.dt(f, 2, n)), d_0p = R(t.dt(f, 3, n));
+    const R w_mm = d_p0 * d_0p;
+    const R w_pm = d_m0 * d_0p;
+    const R w_mp = d_p0 * d_0m;
+    const R w_pp = d_m0 * d_0m;
+    const R normalization = (d_m0 + d_p0) * (d_0m + d_0p);
+    return (R(g[t.idx(f, 3, n)]) * w_pp + R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 2, n)]) * w_mp +
+            R(g[t.idx(f, 0, n)]) * w_mm) /
+           normalization;
+  }
+}
+
+// central_second_derivative, my_p4est_quad_neighbor_nodes_of_node.h:214-220
+template <class R>
+__device__ __forceinline__ R central_second_derivative(R fp, R f0, R fm, R dp, R dm) {
+  return ((fp - f0) / dp + (fm - f0) / dm) * R(2.) / (dp + dm);
+}
+
+template <class R>
+__device__ __forceinline__ bool check_if_zero(R v) { return fabs(v) < R(EPS); }
+
+// correct_naive_second_derivatives, my_p4est_quad_neighbor_nodes_of_node.cpp:44-298
+template <int D, class R>
+__device__ __forceinline__ void correct_naive(const Tables<D> &t, int64_t n, R &Dxx, R &Dyy, R &Dzz) {
+  const R d_m00 = R(t.d(0, n)), d_p00 = R(t.d(1, n)), d_0m0 = R(t.d(2, n)), d_0p0 = R(t.d(3, n));
+  const R m12 = R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)) / d_m00 / (d_p00 + d_m00) +
+                R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)) / d_p00 / (d_p00 + d_m00);
+  const R m21 = R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)) / d_0m0 / (d_0p0 + d_0m0) +
+                R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)) / d_0p0 / (d_0p0 + d_0m0);
+  if constexpr (D == 2) {
+    if (check_if_zero(m12)) {
+      if (!check_if_zero(m21)) Dyy -= m21 * Dxx;
+    } else if (check_if_zero(m21)) {
+      Dxx -= m12 * Dyy;
+    } else {
+      const R idet = R(1.0) / (R(1.0) - m12 * m21);
+      const R nDxx = Dxx, nDyy = Dyy;
+      Dxx = nDxx * idet - nDyy * (m12 * idet);
+      Dyy = nDyy * idet - nDxx * (m21 * idet);
+    }
+  } else {
+    const R d_00m = R(t.d(4, n)), d_00p = R(t.d(5, n));
+    const R m13 = R(t.dt(0, 2, n)) * R(t.dt(0, 3, n)) / d_m00 / (d_p00 + d_m00) +
+                  R(t.dt(1, 2, n)) * R(t.dt(1, 3, n)) / d_p00 / (d_p00 + d_m00);
+    const R m23 = R(t.dt(2, 2, n)) * R(t.dt(2, 3, n)) / d_0m0 / (d_0p0 + d_0m0) +
+                  R(t.dt(3, 2, n)) * R(t.dt(3, 3, n)) / d_0p0 / (d_0p0 + d_0m0);
+    const R m31 = R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)) / d_00m / (d_00p + d_00m) +
+                  R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)) / d_00p / (d_00p + d_00m);
+    const R m32 = R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)) / d_00m / (d_00p + d_00m) +
+                  R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)) / d_00p / (d_00p + d_00m);
+    if (check_if_zero(m12) && check_if_zero(m13)) {
+      if (check_if_zero(m23)) {
+        const bool z21 = !check_if_zero(m21), z31 = !check_if_zero(m31), z32 = !check_if_zero(m32);
+        if (z21) Dyy -= m21 * Dxx;
+        if (z31) Dzz -= m31 * Dxx;
+        if (z32) Dzz -= m32 * Dyy;
+      } else if (check_if_zero(m32)) {
+        const bool z21 = !check_if_zero(m21), z31 = !check_if_zero(m31);
+        if (z31) Dzz -= m31 * Dxx;
+        Dyy -= m23 * Dzz;
+        if (z21) Dyy -= m21 * Dxx;
+      } else {
+        const R idet = R(1.0) / (R(1.0) - m23 * m32);
+        const bool i21 = check_if_zero(m21), i31 = check_if_zero(m31);
+        const R nDyy = Dyy, nDzz = Dzz;
+        Dyy = ((i21 ? nDyy : (nDyy - m21 * Dxx)) - m23 * (i31 ? nDzz : (nDzz - m31 * Dxx))) * idet;
+        Dzz = ((i31 ? nDzz : (nDzz - m31 * Dxx)) - m32 * (i21 ? nDyy : (nDyy - m21 * Dxx))) * idet;
+      }
+    } else if (check_if_zero(m21) && check_if_zero(m23)) {
+      if (check_if_zero(m13)) {
+        const bool z12 = !check_if_zero(m12), z31 = !check_if_zero(m31), z32 = !check_if_zero(m32);
+        if (z12) Dxx -= m12 * Dyy;
+        if (z31) Dzz -= m31 * Dxx;
+        if (z32) Dzz -= m32 * Dyy;
+      } else if (check_if_zero(m31)) {
+        const bool z12 = !check_if_zero(m12), z32 = !check_if_zero(m32);
+        if (z32) Dzz -= m32 * Dyy;
+        Dxx -= m13 * Dzz;
+        if (z12) Dxx -= m12 * Dyy;
+      } else {
+        const R idet = R(1.0) / (R(1.0) - m13 * m31);
+        const bool i12 = check_if_zero(m12), i32 = check_if_zero(m32);
+        const R nDxx = Dxx, nDzz = Dzz;
+        Dxx = ((i12 ? nDxx : (nDxx - m12 * Dyy)) - m13 * (i32 ? nDzz : (nDzz - m32 * Dyy))) * idet;
+        Dzz = ((i32 ? nDzz : (nDzz - m32 * Dyy)) - m31 * (i12 ? nDxx : (nDxx - m12 * Dyy))) * idet;
+      }
+    } else if (check_if_zero(m31) && check_if_zero(m32)) {
+      if (check_if_zero(m12)) {
+        const bool z13 = !check_if_zero(m13), z21 = !check_if_zero(m21), z23 = !check_if_zero(m23);
+        if (z13) Dxx -= m13 * Dzz;
+        if (z21) Dyy -= m21 * Dxx;
+        if (z23) Dyy -= m23 * Dzz;
+      } else if (check_if_zero(m21)) {
+        const bool z13 = !check_if_zero(m13), z23 = !check_if_zero(m23);
+        if (z23) Dyy -= m23 * Dzz;
+        Dxx -= m12 * Dyy;
+        if (z13) Dxx -= m13 * Dzz;
+      } else {
+        const R idet = R(1.0) / (R(1.0) - m12 * m21);
+        const bool i13 = check_if_zero(m13), i23 = check_if_zero(m23);
+        const R nDxx = Dxx, nDyy = Dyy;
+        Dxx = ((i13 ? nDxx : (nDxx - m13 * Dzz)) - m12 * (i23 ? nDyy : (nDyy - m23 * Dzz))) * idet;
+        Dyy = ((i23 ? nDyy : (nDyy - m23 * Dzz)) - m21 * (i13 ? nDxx : (nDxx - m13 * Dzz))) * idet;
+      }
+    } else {
+      const R idet =
+          R(1.0) / (R(1.0) + m12 * m23 * m31 + m13 * m21 * m32 - m12 * m21 - m13 * m31 - m23 * m32);
+      const R nDxx = Dxx, nDyy = Dyy, nDzz = Dzz;
+      Dxx = (nDxx * (idet * (R(1.0) - m23 * m32)) + nDyy * (idet * (m32 * m13 - m12)) +
+             nDzz * (idet * (m12 * m23 - m13)));
+      Dyy = (nDxx * (idet * (m31 * m23 - m21)) + nDyy * (idet * (R(1.0) - m31 * m13)) +
+             nDzz * (idet * (m21 * m13 - m23)));
+      Dzz = (nDxx * (idet * (m21 * m32 - m31)) + nDyy * (idet * (m31 * m12 - m32)) +
+             nDzz * (idet * (R(1.0) - m21 * m12)));
+    }
+  }
+}
+
+// laplace, my_p4est_quad_neighbor_nodes_of_node.h:615-635 (one field)
+template <int D, class R, class G>
+__device__ __forceinline__ void laplace(const Tables<D> &t, int64_t n, const G *__restrict__ g, R &f000, R *fnb,
+                                        R *dd) {
+  f000 = R(g[n]);
+#pragma unroll
+  for (int f = 0; f < 2 * D; ++f) fnb[f] = face_linear<D, R>(t, f, n, g);
+  dd[0] = central_second_derivative(fnb[1], f000, fnb[0], R(t.d(1, n)), R(t.d(0, n)));
+  dd[1] = central_second_derivative(fnb[3], f000, fnb[2], R(t.d(3, n)), R(t.d(2, n)));
+  R dzz = R(0);
+  if constexpr (D == 3) dzz = central_second_derivative(fnb[5], f000, fnb[4], R(t.d(5, n)), R(t.d(4, n)));
+  correct_naive<D, R>(t, n, dd[0], dd[1], dzz);
+  if constexpr (D == 3) dd[2] = dzz;
+}
+
+// ngbd_with_quadratic_interpolation, my_p4est_quad_neighbor_nodes_of_node.h:1514-1545
+template <int D, class R, class G>
+__device__ __forceinline__ void ngbd_quadratic(const Tables<D> &t, int64_t n, const G *__restrict__ g, R &f000,
+                                               R *fnb, R *dd) {
+  laplace<D, R>(t, n, g, f000, fnb, dd);
+  const R h = R(0.5);
+  if constexpr (D == 2) {
+    fnb[0] -= (h * R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)) * dd[1]);
+    fnb[1] -= (h * R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)) * dd[1]);
+    fnb[2] -= (h * R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)) * dd[0]);
+    fnb[3] -= (h * R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)) * dd[0]);
+  } else {
+    fnb[0] -= (h * R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)) * dd[1] + h * R(t.dt(0, 2, n)) * R(t.dt(0, 3, n)) * dd[2]);
+    fnb[1] -= (h * R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)) * dd[1] + h * R(t.dt(1, 2, n)) * R(t.dt(1, 3, n)) * dd[2]);
+    fnb[2] -= (h * R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)) * dd[0] + h * R(t.dt(2, 2, n)) * R(t.dt(2, 3, n)) * dd[2]);
+    fnb[3] -= (h * R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)) * dd[0] + h * R(t.dt(3, 2, n)) * R(t.dt(3, 3, n)) * dd[2]);
+    fnb[4] -= (h * R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)) * dd[0] + h * R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)) * dd[1]);
+    fnb[5] -= (h * R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)) * dd[0] + h * R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)) * dd[1]);
+  }
+}
+
+}  // namespace dev
+}  // namespace casl

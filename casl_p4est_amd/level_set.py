@@ -1,0 +1,205 @@
+"""my_p4est_level_set_t reinitialization entry points on the GPU.
+
+Mirrors src/my_p4est_level_set.h:385-613 (names, argument meaning and error
+behaviour): every reinitialize_* method forwards to
+reinitialize_within_range_of_phi_0 (src/my_p4est_level_set.cpp:290-557),
+which here is one call into the native library (casl_reinit).  phi is
+updated in place.  It may be a torch float64 CUDA tensor (kept on the
+device) or a float64 numpy array (copied to the device and back).
+
+Errors: invalid orders raise ValueError (the reference throws
+std::invalid_argument); any other native failure raises CaslError.
+"""
+import ctypes
+import sys
+
+import numpy as np
+
+from . import _lib as L
+
+DBL_MAX = sys.float_info.max
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("casl_p4est_amd needs a HIP device for reinitialization")
+    return torch
+
+
+def _stream_ptr(torch):
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceStencil:
+    """Device copy of the flattened stencils (casl_stencil_create)."""
+
+    def __init__(self, tables):
+        torch = _torch()
+        torch.cuda.init()
+        self.tables = tables
+        self.dim = tables.dim
+        self.n_nodes = tables.n_nodes
+        h = ctypes.c_void_p()
+        s = tables.struct()
+        L.call("casl_stencil_create", ctypes.byref(s), ctypes.byref(h))
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib().casl_stencil_destroy(h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def _check_field(self, t, name):
+        torch = _torch()
+        if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.float64:
+            raise TypeError(f"{name} must be a float64 CUDA tensor")
+        if not t.is_contiguous():
+            raise ValueError(f"{name} must be contiguous")
+
+    def second_derivatives(self, f, fdd=None):
+        torch = _torch()
+        self._check_field(f, "f")
+        if f.numel() != self.n_nodes:
+            raise ValueError("f must hold one value per node")
+        if fdd is None:
+            fdd = torch.empty((self.dim, self.n_nodes), dtype=torch.float64, device=f.device)
+        self._check_field(fdd, "fdd")
+        if fdd.numel() != self.dim * self.n_nodes:
+            raise ValueError("fdd must hold dim values per node")
+        L.call("casl_second_derivatives", self._h, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(fdd.data_ptr()),
+               _stream_ptr(torch))
+        return fdd
+
+    def reinit(self, phi, order_space, order_time, n_iter, limit_high, limit_low, precision, mask=None):
+        torch = _torch()
+        self._check_field(phi, "phi")
+        if phi.numel() != self.n_nodes:
+            raise ValueError("phi must hold one value per node")
+        st = _stream_ptr(torch)
+        if mask is None:
+            L.call("casl_reinit", self._h, ctypes.c_void_p(phi.data_ptr()), int(order_space), int(order_time),
+                   int(n_iter), float(limit_high), float(limit_low), int(precision), st)
+        else:
+            if mask.dtype != torch.uint8 or mask.device.type != "cuda" or mask.numel() != self.n_nodes:
+                raise TypeError("mask must be a uint8 CUDA tensor with one entry per node")
+            L.call("casl_reinit_masked", self._h, ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(mask.data_ptr()),
+                   int(n_iter), float(limit_high), float(limit_low), int(precision), st)
+        return phi
+
+    def band_linf(self, a, b, ref, band):
+        torch = _torch()
+        for t, n in ((a, "a"), (b, "b"), (ref, "ref")):
+            self._check_field(t, n)
+        out = ctypes.c_double()
+        L.call("casl_band_linf", self._h, ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()),
+               ctypes.c_void_p(ref.data_ptr()), float(band), ctypes.byref(out), _stream_ptr(torch))
+        return out.value
+
+
+class LevelSet:
+    """my_p4est_level_set_t(ngbd): reinitialization of node-sampled level sets.
+
+    precision: "fp64" (default; every operation in FP64) or "mixed" (FP32 bulk
+    updates and second derivatives, FP64 subcell roots and norms).
+    """
+
+    def __init__(self, ngbd, precision="fp64"):
+        self.ngbd = ngbd
+        if ngbd.tables is None:
+            ngbd.init_neighbors()
+        self.precision = precision
+
+    def _precision_code(self, precision):
+        p = self.precision if precision is None else precision
+        if p in ("fp64", L.CASL_PRECISION_FP64):
+            return L.CASL_PRECISION_FP64
+        if p in ("mixed", L.CASL_PRECISION_MIXED):
+            return L.CASL_PRECISION_MIXED
+        raise ValueError(f"unknown precision {p!r}")
+
+    def reinitialize_within_range_of_phi_0(self, phi, order_space, order_pseudotime, phi_0_limit_high,
+                                           phi_0_limit_low, number_of_iterations, mask=None, precision=None):
+        if order_space not in (1, 2) or order_pseudotime not in (1, 2):
+            raise ValueError("my_p4est_level_set_t::reinitialize_within_range_of_phi_0: valid orders of "
+                             "accuracy for space and pseudotime are 1 and 2 only!")
+        code = self._precision_code(precision)
+        dev = self.ngbd.device()
+        if isinstance(phi, np.ndarray):
+            torch = _torch()
+            if phi.dtype != np.float64 or phi.shape != (dev.n_nodes,):
+                raise TypeError("phi must be a float64 array with one value per node")
+            t = torch.from_numpy(phi).cuda()
+            m = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).cuda() if mask is not None else None
+            dev.reinit(t, order_space, order_pseudotime, number_of_iterations, phi_0_limit_high, phi_0_limit_low,
+                       code, m)
+            phi[:] = t.cpu().numpy()
+            return phi
+        try:
+            return dev.reinit(phi, order_space, order_pseudotime, number_of_iterations, phi_0_limit_high,
+                              phi_0_limit_low, code, mask)
+        except L.CaslError as e:
+            if e.code == L.CASL_ERR_INVALID_ARGUMENT:
+                raise ValueError(str(e)) from e
+            raise
+
+    # --- the public entry points of my_p4est_level_set.h:434-613
+    def reinitialize_1st_order(self, phi, number_of_iterations=50, limit_absolute_value_phi_0=DBL_MAX):
+        return self.reinitialize_within_range_of_phi_0(phi, 1, 1, limit_absolute_value_phi_0,
+                                                       -limit_absolute_value_phi_0, number_of_iterations)
+
+    def reinitialize_1st_order_above_threshold(self, phi, threshold, number_of_iterations=50,
+                                               upper_limit_phi_0=DBL_MAX):
+        return self.reinitialize_within_range_of_phi_0(phi, 1, 1, upper_limit_phi_0, threshold, number_of_iterations)
+
+    def reinitialize_2nd_order(self, phi, number_of_iterations=50, limit_absolute_value_phi_0=DBL_MAX,
+                               precision=None):
+        return self.reinitialize_within_range_of_phi_0(phi, 2, 2, limit_absolute_value_phi_0,
+                                                       -limit_absolute_value_phi_0, number_of_iterations,
+                                                       precision=precision)
+
+    def reinitialize_2nd_order_with_mask(self, phi, mask, number_of_masked=0, number_of_iterations=50,
+                                         limit_absolute_value_phi_0=DBL_MAX, precision=None):
+        """mask: nodes with mask == 0 keep their current value (number_of_masked is
+        the reference's reservation hint and is not needed here)."""
+        torch = _torch()
+        if isinstance(mask, np.ndarray) or not isinstance(mask, torch.Tensor) or mask.dtype != torch.uint8:
+            mask = torch.as_tensor(np.asarray(mask if not isinstance(mask, torch.Tensor) else mask.cpu()) != 0,
+                                   dtype=torch.uint8)
+        if isinstance(phi, torch.Tensor):
+            mask = mask.to(phi.device)
+        return self.reinitialize_within_range_of_phi_0(phi, 2, 2, limit_absolute_value_phi_0,
+                                                       -limit_absolute_value_phi_0, number_of_iterations,
+                                                       mask=mask, precision=precision)
+
+    def reinitialize_2nd_order_above_threshold(self, phi, threshold, number_of_iterations=50,
+                                               upper_limit_phi_0=DBL_MAX):
+        return self.reinitialize_within_range_of_phi_0(phi, 2, 2, upper_limit_phi_0, threshold, number_of_iterations)
+
+    def reinitialize_2nd_order_time_1st_order_space(self, phi, number_of_iterations=50,
+                                                    limit_absolute_value_phi_0=DBL_MAX):
+        return self.reinitialize_within_range_of_phi_0(phi, 1, 2, limit_absolute_value_phi_0,
+                                                       -limit_absolute_value_phi_0, number_of_iterations)
+
+    def reinitialize_2nd_order_time_1st_order_space_above_threshold(self, phi, threshold, number_of_iterations=50,
+                                                                    upper_limit_phi_0=DBL_MAX):
+        return self.reinitialize_within_range_of_phi_0(phi, 1, 2, upper_limit_phi_0, threshold, number_of_iterations)
+
+    def reinitialize_1st_order_time_2nd_order_space(self, phi, number_of_iterations=50,
+                                                    limit_absolute_value_phi_0=DBL_MAX, precision=None):
+        return self.reinitialize_within_range_of_phi_0(phi, 2, 1, limit_absolute_value_phi_0,
+                                                       -limit_absolute_value_phi_0, number_of_iterations,
+                                                       precision=precision)
+
+    def reinitialize_1st_order_time_2nd_order_space_above_threshold(self, phi, threshold, number_of_iterations=50,
+                                                                    upper_limit_phi_0=DBL_MAX):
+        return self.reinitialize_within_range_of_phi_0(phi, 2, 1, upper_limit_phi_0, threshold, number_of_iterations)
+
+
+# the reference's class names
+my_p4est_level_set_t = LevelSet

@@ -1,0 +1,216 @@
+/*
+ * casl_reinit.h -- C ABI of the MI355X level-set reinitialization library.
+ *
+ * Plain pointers and sizes only.  Every call returns an int status (CASL_OK
+ * or a negative CASL_ERR_* code); no C++ exception crosses this boundary.
+ * Device pointers are HIP device allocations on the current device; `stream`
+ * is a hipStream_t passed as void* (NULL = the null stream).
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   - my_p4est_level_set_t::reinitialize_{1st_order, 2nd_order,
+ *     1st_order_time_2nd_order_space, 2nd_order_time_1st_order_space}
+ *     and their *_above_threshold forms        src/my_p4est_level_set.h:434-613
+ *     all of which forward to reinitialize_within_range_of_phi_0
+ *                                              src/my_p4est_level_set.cpp:290-557
+ *   - my_p4est_level_set_t::reinitialize_2nd_order_with_mask
+ *                                              src/my_p4est_level_set.h:487-530
+ *   - my_p4est_node_neighbors_t::second_derivatives_central(Vec, Vec[DIM])
+ *                                              src/my_p4est_node_neighbors.h:561-565,
+ *                                              src/my_p4est_node_neighbors.cpp:1422-1641
+ *   - my_p4est_node_neighbors_t::init_neighbors (stencil storage)
+ *                                              src/my_p4est_node_neighbors.cpp:33-61
+ *   - host-side grid construction that the reference takes from p4est:
+ *     my_p4est_new + my_p4est_refine(refine_levelset_cf)
+ *                                              src/my_p4est_refine_coarsen.cpp:17-89,
+ *                                              examples/reinitialization/main_2d.cpp:165-173
+ *     my_p4est_nodes_new                       src/my_p4est_nodes.c:383-446, 614-671
+ *     construct_neighbors                      src/my_p4est_node_neighbors.cpp:114-1201
+ *     sample_cf_on_nodes                       src/my_p4est_utils.cpp:2129-2170
+ */
+#ifndef CASL_REINIT_H
+#define CASL_REINIT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+#define CASL_OK                     0
+#define CASL_ERR_INVALID_ARGUMENT  -1 /* reference: std::invalid_argument   */
+#define CASL_ERR_NO_DEVICE         -2 /* no HIP device / library not usable */
+#define CASL_ERR_HIP               -3 /* a HIP runtime call failed          */
+#define CASL_ERR_OUT_OF_MEMORY     -4
+#define CASL_ERR_INEXACT_DISTANCES -5 /* stencil distances not dyadic       */
+#define CASL_ERR_INVALID_STENCIL   -6 /* index out of range, bad wall arm   */
+#define CASL_ERR_NOT_BUILT         -7 /* forest step called out of order    */
+
+/* Last error message of the calling thread (static storage, never NULL). */
+const char *casl_last_error(void);
+
+/* ------------------------------------------------------- precision modes */
+#define CASL_PRECISION_FP64  0 /* every operation in FP64                 */
+#define CASL_PRECISION_MIXED 1 /* FP32 bulk Godunov/ENO updates and second
+                                  derivatives; FP64 subcell root finding
+                                  and FP64 convergence norm               */
+
+/* ---------------------------------------------------- flattened stencils
+ * One record per node, structure-of-arrays, field for field the per-node
+ * struct quad_neighbor_nodes_of_node_t (my_p4est_quad_neighbor_nodes_of_node.h:1191-1225).
+ * Faces are ordered m00, p00, 0m0, 0p0[, 00m, 00p]; the corners of a face
+ * are ordered mm, pm[, mp, pp] over its two transverse axes (lower axis
+ * first); the transverse distances of a face are ordered m0, p0[, 0m, 0p].
+ * With F = 2*dim, C = 2^(dim-1), T = 2*(dim-1), N = n_nodes:
+ *   nbr [(f*C + c)*N + n]  = node_{f}_{c}      (int32 node index)
+ *   d   [f*N + n]          = d_{f}             (negative on a wall arm,
+ *                                               construct_neighbors :609-719)
+ *   d_tr[(f*T + t)*N + n]  = d_{f}_{t}
+ *   wall[n]                bit f set <=> is_node_{x,y,z}{m,p}Wall(node)
+ *                                       (my_p4est_utils.cpp:1815-1905)
+ */
+typedef struct casl_flat_stencil {
+  int32_t        dim;     /* 2 or 3 */
+  int64_t        n_nodes;
+  const int32_t *nbr;
+  const double  *d;
+  const double  *d_tr;
+  const uint8_t *wall;
+  double         zero_distance_threshold; /* EPS*min(tree dimensions),
+                                             my_p4est_level_set.h:388 */
+} casl_flat_stencil;
+
+/* ------------------------------------------------- device stencil handle */
+typedef struct casl_stencil casl_stencil_t;
+
+/* Validates the host tables and uploads them to the current device
+ * (replaces init_neighbors' cached qnnn vector).  All distances must be
+ * exactly representable in FP32 (true for every dyadic tree of power-of-two
+ * size), otherwise CASL_ERR_INEXACT_DISTANCES. */
+int casl_stencil_create(const casl_flat_stencil *tables, casl_stencil_t **out);
+int casl_stencil_destroy(casl_stencil_t *st);
+int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes);
+
+/* second_derivatives_central: fdd[a*N + n] = d^2 f / dx_a^2 at node n.
+ * f and fdd are FP64 device arrays. */
+int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd,
+                            void *stream);
+
+/* reinitialize_within_range_of_phi_0(phi, order_space, order_pseudotime,
+ * phi_0_limit_high, phi_0_limit_low, number_of_iterations): phi is an FP64
+ * device array of n_nodes values, updated in place.  Orders outside {1,2}
+ * -> CASL_ERR_INVALID_ARGUMENT (reference throws std::invalid_argument).
+ * Pass +/-DBL_MAX limits for the plain reinitialize_* entry points. */
+int casl_reinit(casl_stencil_t *st, double *phi, int order_space,
+                int order_pseudotime, int n_iter, double limit_high,
+                double limit_low, int precision, void *stream);
+
+/* reinitialize_2nd_order_with_mask: nodes with mask[n] == 0 keep their
+ * current value at every stage (orders 2/2).  mask is a device uint8 array. */
+int casl_reinit_masked(casl_stencil_t *st, double *phi, const uint8_t *mask,
+                       int n_iter, double limit_high, double limit_low,
+                       int precision, void *stream);
+
+/* L-infinity norm of (a - b) over nodes with |ref| < band (FP64 reduction on
+ * the device; used for the mixed-precision error report).  Result on host. */
+int casl_band_linf(casl_stencil_t *st, const double *a, const double *b,
+                   const double *ref, double band, double *result, void *stream);
+
+/* --------------------------------------------------------- kernel timing
+ * When enabled, every kernel launch of the library is bracketed by HIP
+ * events recorded on its launch stream.  casl_profile_query returns the
+ * summed event time (ms) and the launch count of one kernel ("k_dd",
+ * "k_prep", "k_stage", "k_batch") since the last reset; it synchronizes the
+ * recorded events.  Off by default (no events, no overhead). */
+int casl_profile_enable(int enable);
+int casl_profile_reset(void);
+int casl_profile_query(const char *kernel, double *total_ms, int64_t *launches);
+
+/* ------------------------------------------------- batched small trees
+ * Many independent trees reinitialized by one persistent kernel.  The trees'
+ * stencils are given as one concatenated casl_flat_stencil per tree; phi is
+ * the concatenation of the trees' node values in the same order. */
+typedef struct casl_batch casl_batch_t;
+int casl_batch_create(const casl_flat_stencil *trees, int32_t n_trees,
+                      casl_batch_t **out);
+int casl_batch_destroy(casl_batch_t *b);
+int casl_batch_info(const casl_batch_t *b, int64_t *total_nodes,
+                    int32_t *n_trees, int32_t *max_tree_nodes);
+/* Launch schedule: threads per block and nodes per thread (0 = defaults). */
+int casl_batch_set_schedule(casl_batch_t *b, int32_t block_threads,
+                            int32_t nodes_per_thread);
+int casl_batch_get_schedule(const casl_batch_t *b, int32_t *block_threads,
+                            int32_t *nodes_per_thread);
+/* Times every supported launch shape on a copy of phi (phi is not modified),
+ * keeps the fastest as the batch's schedule and reports it. */
+int casl_batch_autotune(casl_batch_t *b, const double *phi, int n_iter,
+                        int precision, void *stream, int32_t *best_block,
+                        int32_t *best_nodes_per_thread, float *best_ms);
+/* reinitialize_2nd_order (orders 2/2, limits +/-DBL_MAX) of every tree. */
+int casl_batch_reinit(casl_batch_t *b, double *phi, int n_iter,
+                      int precision, void *stream);
+/* Same, recorded once into a hipGraph and replayed (no per-call launches
+ * or allocations after the first call with a given n_iter/precision). */
+int casl_batch_reinit_graph(casl_batch_t *b, double *phi, int n_iter,
+                            int precision, void *stream);
+
+/* ------------------------------------------------ host forest (p4est role)
+ * A single tree on the box [box_min, box_max] (one brick tree, non periodic). */
+typedef struct casl_forest casl_forest_t;
+
+/* Level-set descriptions for refinement and sampling. */
+#define CASL_LS_CALLBACK  0 /* fn(xyz, ctx)                                  */
+#define CASL_LS_STAR      1 /* 2D: (|x-c| - (r0 + A cos(p*theta)))*(1 + k|x-c|^2)
+                               params: cx cy r0 A p k                       */
+#define CASL_LS_SPHERES   2 /* 3D: min_i(|x-c_i| - r_i)*(1 + m sin(2 pi x) cos(2 pi y))
+                               params: m, then n_items x (cx cy cz r)        */
+#define CASL_LS_GYROID    3 /* 3D: (sin(w x)cos(w y) + sin(w y)cos(w z)
+                                    + sin(w z)cos(w x) - c)*(1 + e x y z)
+                               params: w c e                                 */
+#define CASL_LS_SPHERE    4 /* 2D/3D signed distance |x-c| - r; params: c.., r */
+#define CASL_LS_PLANE     5 /* 2D/3D a.x + b; params: a.., b                 */
+#define CASL_LS_MODULATED 6 /* 2D: (|x-c| - r)*(1 + q sin(5x) cos(4y)) (the
+                               reference reinitialization example's phi0);
+                               params: cx cy r q                             */
+
+typedef struct casl_levelset {
+  int32_t       kind;
+  int32_t       n_items;  /* spheres: number of spheres */
+  const double *params;
+  double (*fn)(const double *xyz, void *ctx);
+  void         *ctx;
+} casl_levelset;
+
+int casl_levelset_eval(const casl_levelset *ls, int dim, int64_t n,
+                       const double *xyz /* [n][dim] */, double *out);
+
+int casl_forest_new(int dim, const double *box_min, const double *box_max,
+                    casl_forest_t **out);
+int casl_forest_destroy(casl_forest_t *f);
+/* refine_levelset_cf with splitting_criteria_cf_t(min_lvl, max_lvl, phi, lip,
+ * band), applied max_lvl times (non-recursive refinement, no 2:1 balance). */
+int casl_forest_refine_levelset(casl_forest_t *f, int min_lvl, int max_lvl,
+                                double lip, double band,
+                                const casl_levelset *phi);
+/* my_p4est_nodes_new: canonical clamped corner nodes in Morton order. */
+int casl_forest_build_nodes(casl_forest_t *f);
+int casl_forest_sizes(const casl_forest_t *f, int64_t *n_leaves,
+                      int64_t *n_nodes, int32_t *max_level);
+/* leaves: xyz [L][dim] logical coords, level [L], local_nodes [L][2^dim] */
+int casl_forest_get_leaves(const casl_forest_t *f, int32_t *xyz,
+                           int8_t *level, int32_t *local_nodes);
+/* nodes: logical (unclamped) coords [N][dim] and physical coords [N][dim] */
+int casl_forest_get_nodes(const casl_forest_t *f, int32_t *ixyz,
+                          double *xyz);
+/* sample_cf_on_nodes */
+int casl_forest_sample(const casl_forest_t *f, const casl_levelset *phi,
+                       double *out);
+/* init_neighbors / construct_neighbors for every node.  The table pointers
+ * stay valid until the forest is destroyed. */
+int casl_forest_init_neighbors(casl_forest_t *f, casl_flat_stencil *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CASL_REINIT_H */

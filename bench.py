@@ -105,7 +105,7 @@ def kernel_profile(fn, torch):
     fn()
     torch.cuda.synchronize()
     out = {}
-    for k in ("k_dd", "k_prep", "k_stage", "k_batch"):
+    for k in ("k_dd", "k_prep", "k_stage", "k_tile_stage", "k_batch"):
         ms, cnt = ctypes.c_double(), ctypes.c_int64()
         L.call("casl_profile_query", k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
         if cnt.value:
@@ -142,6 +142,8 @@ def main():
     ap.add_argument("--lmax", type=int, default=0, help="reduce max level (quick runs only)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--path", default="tiled", choices=["tiled", "two_pass"],
+                    help="stage kernels: fused tiled kernel (default) or grid-wide two-pass")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -157,7 +159,8 @@ def main():
     N, dim, iters = W["ngbd"].num_nodes, W["dim"], W["iters"]
     h = float(np.min(W["forest"].dxyz_min()))
     lsr = C.LevelSet(W["ngbd"])
-    W["ngbd"].device()  # upload tables before timing
+    W["ngbd"].device().set_path(args.path)  # uploads tables before timing
+    log(f"[bench] stencil tiles: {W['ngbd'].device().tile_info()}")
     phi0 = torch.from_numpy(W["phi0"]).to(dev)
     phi = torch.empty_like(phi0)
 
@@ -182,7 +185,8 @@ def main():
         kp = kernel_profile(step(prec), torch)
         kernels = {}
         for k, (ms, cnt) in kp.items():
-            per = {"k_stage": stage_bytes(dim, w) * N, "k_dd": dd_bytes(dim, w) * N}.get(k)
+            per = {"k_stage": stage_bytes(dim, w) * N, "k_tile_stage": stage_bytes(dim, w) * N,
+                   "k_dd": dd_bytes(dim, w) * N}.get(k)
             avg = ms / cnt
             kernels[k] = dict(avg_ms=avg, launches=cnt, total_ms=ms,
                               alg_bytes_per_launch=per,
@@ -217,7 +221,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 bulk + f64 subcell/norm",
         "data": "synthetic",
-        "config": {"workload": W["desc"], "nodes": N, "iterations": iters, "h_min": h},
+        "config": {"workload": W["desc"], "nodes": N, "iterations": iters, "h_min": h, "path": args.path},
         "fp64": {"value": rate_fp64, "unit": "node-iters/s", "ms_per_step": 1e3 * t_fp64 / args.steps},
         "mixed_over_fp64": rate_mixed / rate_fp64,
         "linf_err_band": err,

@@ -25,6 +25,9 @@ HIP_FLAGS = [
     # strict IEEE evaluation: the FP64 path must reproduce the reference's
     # operation order exactly
     "-ffp-contract=off",
+    # FP32 division/sqrt may be approximate: only the mixed path computes in
+    # FP32 and its accuracy bar is relative to FP64 (1e-3 h); FP64 is unaffected
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", INCLUDE, "-I", CSRC,
@@ -32,7 +35,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", INCLUDE, "-I", CSRC]
 
 HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip"]
-HOST_SOURCES = ["forest.cpp"]
+HOST_SOURCES = ["forest.cpp", "tiles.cpp"]
 
 
 def _run(cmd, verbose):

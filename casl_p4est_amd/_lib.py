@@ -18,6 +18,9 @@ CASL_ERR_INEXACT_DISTANCES = -5
 CASL_ERR_INVALID_STENCIL = -6
 CASL_ERR_NOT_BUILT = -7
 
+CASL_PATH_TILED = 0
+CASL_PATH_TWO_PASS = 1
+
 CASL_PRECISION_FP64 = 0
 CASL_PRECISION_MIXED = 1
 
@@ -74,6 +77,9 @@ SIGNATURES = {
     "casl_stencil_create": (ctypes.c_int, [ctypes.POINTER(FlatStencil), ctypes.POINTER(ctypes.c_void_p)]),
     "casl_stencil_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "casl_stencil_info": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p]),
+    "casl_stencil_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "casl_stencil_tile_info": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p, c_int64_p, c_int64_p,
+                                              c_int64_p]),
     "casl_second_derivatives": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "casl_reinit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),

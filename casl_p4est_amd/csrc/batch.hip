@@ -280,7 +280,7 @@ int casl_batch_create(const casl_flat_stencil *trees, int32_t n_trees, casl_batc
     all.d_tr = dtr.data();
     all.wall = wall.data();
     all.zero_distance_threshold = zdt;
-    upload_tables(&all, &B->st);
+    upload_tables(&all, &B->st, false);
     B->n_trees = n_trees;
     B->max_tree = max_tree;
     std::vector<int32_t> order(n_trees);

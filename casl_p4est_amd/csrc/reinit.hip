@@ -24,6 +24,7 @@
 #include <vector>
 
 #include "reinit_device.hpp"
+#include "tiles_device.hpp"
 
 using namespace casl;
 using namespace casl::dev;
@@ -178,7 +179,54 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
     stage_launch(in, out, avg);
     prof_end("k_stage", s);
   };
+  const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
+  TileArgs<R> ta{};
+  if (tiled) {
+    ta.tile_start = st->t_tile_start.p;
+    ta.halo_start = st->t_halo_start.p;
+    ta.h1count = st->t_h1count.p;
+    ta.halo = st->t_halo.p;
+    ta.rec_start = st->t_rec_start.p;
+    ta.rec = reinterpret_cast<const TileRec *>(st->t_rec.p);
+    ta.rec_slot = st->t_rec_slot.p;
+    ta.ext = st->t_ext.p;
+    for (int a = 0; a < 3; ++a) {
+      ta.geom.base[a] = R(st->geom_base[a]);
+      ta.geom.unit[a] = R(st->geom_unit[a]);
+    }
+    ta.flags = st->flags.p;
+    ta.sfix = sfix;
+    ta.phi0 = st->phi0.p;
+    ta.N = N;
+  }
+  auto tile_stage = [&](const R *in, R *out, bool avg) {
+    ta.phin = in;
+    ta.out = out;
+    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK);
+    prof_begin("k_tile_stage", s);
+    if (so) {
+      if (avg)
+        hipLaunchKernelGGL((k_tile_stage<D, R, true, true>), g, b, 0, s, ta);
+      else
+        hipLaunchKernelGGL((k_tile_stage<D, R, true, false>), g, b, 0, s, ta);
+    } else {
+      if (avg)
+        hipLaunchKernelGGL((k_tile_stage<D, R, false, true>), g, b, 0, s, ta);
+      else
+        hipLaunchKernelGGL((k_tile_stage<D, R, false, false>), g, b, 0, s, ta);
+    }
+    prof_end("k_tile_stage", s);
+  };
   for (int it = 0; it < n_iter; ++it) {
+    if (tiled) {
+      tile_stage(cur, nxt, false);
+      if (order_time > 1)
+        tile_stage(nxt, cur, true);
+      else
+        std::swap(cur, nxt);
+      HIP_CHECK(hipGetLastError());
+      continue;
+    }
     if (so) dd_pass(cur);
     stage(cur, nxt, false);  // phi_n -> phi_np1
     if (order_time > 1) {
@@ -209,7 +257,87 @@ void validate_tables(const casl_flat_stencil *tab) {
 
 namespace casl {
 
-void upload_tables(const casl_flat_stencil *tab, casl_stencil *st) {
+// Builds and uploads the tiled encoding; on failure the stencil keeps only
+// the grid-wide tables (two-pass kernels) and records why.
+void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
+  st->tiled = false;
+  TileHost H;
+  try {
+    build_tiles(*tab, H);
+  } catch (const std::exception &e) {
+    st->tile_error = e.what();
+    return;
+  }
+  // independent bounds check of every record before anything reaches the GPU
+  {
+    const int D = tab->dim, F = 2 * D, C = 1 << (D - 1), W = (D == 2 ? 3 : 6);
+    const int LMAX = D == 2 ? TileLimits<2>::LMAX : TileLimits<3>::LMAX;
+    const int DMAX = D == 2 ? TileLimits<2>::DMAX : TileLimits<3>::DMAX;
+    const size_t n_ext = H.ext.size() / size_t(F * W);
+    const int64_t nt = int64_t(H.tile_start.size()) - 1;
+    if (H.tile_start.front() != 0 || H.tile_start.back() != tab->n_nodes)
+      throw std::runtime_error("tile ranges do not cover the nodes");
+    for (int64_t t = 0; t < nt; ++t) {
+      const int nT = H.tile_start[t + 1] - H.tile_start[t];
+      const int nH = H.halo_start[t + 1] - H.halo_start[t];
+      const int nH1 = H.h1count[t];
+      const int nR = H.rec_start[t + 1] - H.rec_start[t];
+      if (nT <= 0 || nH1 < 0 || nH1 > nH || nT + nH > LMAX || nT + nH1 > DMAX || nR != nT + nH1)
+        throw std::runtime_error("tile exceeds its LDS limits");
+      for (int k = 0; k < nH; ++k) {
+        const int32_t g = H.halo_idx[size_t(H.halo_start[t]) + k];
+        if (g < 0 || g >= tab->n_nodes) throw std::runtime_error("halo node out of range");
+      }
+      std::vector<char> seen(size_t(nR), 0);
+      for (int j = 0; j < nR; ++j) {
+        const TileRec &rr = H.rec[size_t(H.rec_start[t]) + j];
+        const int slot = H.rec_slot[size_t(H.rec_start[t]) + j];
+        // the first nT records are the tile's own nodes (slots < nT), the rest
+        // its H1 nodes (slots nT .. nT + nH1 - 1), each slot exactly once
+        if ((j < nT) ? (slot >= nT) : (slot < nT || slot >= nT + nH1)) throw std::runtime_error("record slot out of range");
+        if (seen[size_t(slot)]++) throw std::runtime_error("record slot repeated");
+        // pass A of any record reads phi slots < nT + nH; pass B of a tile
+        // node also reads second-derivative slots < nT + nH1 at the same corners
+        const int bound = (j < nT) ? nT + nH1 : nT + nH;
+        if (rr.geo & GEO_IRREGULAR) {
+          const size_t id = size_t(rr.idx[0]) | (size_t(rr.idx[1]) << 16);
+          if (id >= n_ext) throw std::runtime_error("extension id out of range");
+          for (int f = 0; f < F; ++f)
+            for (int c = 0; c < C; ++c)
+              if (H.ext[id * F * W + size_t(f) * W + c] >= bound) throw std::runtime_error("corner slot out of range");
+        } else {
+          for (int f = 0; f < F; ++f)
+            if (rr.idx[f] >= bound) throw std::runtime_error("corner slot out of range");
+        }
+      }
+    }
+  }
+  auto up = [](auto &buf, const auto &vec) {
+    using T = typename std::decay_t<decltype(vec)>::value_type;
+    buf.ensure(vec.size());
+    HIP_CHECK(hipMemcpy(buf.p, vec.data(), vec.size() * sizeof(T), hipMemcpyHostToDevice));
+  };
+  up(st->t_tile_start, H.tile_start);
+  up(st->t_halo_start, H.halo_start);
+  up(st->t_h1count, H.h1count);
+  up(st->t_halo, H.halo_idx.empty() ? std::vector<int32_t>(1, 0) : H.halo_idx);
+  up(st->t_rec_start, H.rec_start);
+  up(st->t_rec_slot, H.rec_slot);
+  st->t_rec.ensure(H.rec.size());
+  HIP_CHECK(hipMemcpy(st->t_rec.p, H.rec.data(), H.rec.size() * sizeof(TileRec), hipMemcpyHostToDevice));
+  up(st->t_ext, H.ext.empty() ? std::vector<uint16_t>(8, 0) : H.ext);
+  st->n_tiles = int64_t(H.tile_start.size()) - 1;
+  st->n_halo = int64_t(H.halo_idx.size());
+  st->n_records = int64_t(H.rec.size());
+  st->n_irregular = H.n_irregular;
+  for (int a = 0; a < 3; ++a) {
+    st->geom_base[a] = H.size[a];
+    st->geom_unit[a] = H.size[a] * std::ldexp(1.0, -H.lmax);
+  }
+  st->tiled = true;
+}
+
+void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles) {
   validate_tables(tab);
   require_device();
   const int D = tab->dim;
@@ -240,6 +368,7 @@ void upload_tables(const casl_flat_stencil *tab, casl_stencil *st) {
   HIP_CHECK(hipMemcpy(st->nbr.p, tab->nbr, nidx * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(st->dist.p, dist.data(), dist.size() * sizeof(float), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(st->wall.p, tab->wall, N * sizeof(uint8_t), hipMemcpyHostToDevice));
+  if (with_tiles) upload_tile_tables(tab, st);
 }
 
 // ------------------------------------------------------------ kernel timing
@@ -335,7 +464,7 @@ int casl_stencil_create(const casl_flat_stencil *tab, casl_stencil_t **out) {
     if (!out) throw std::invalid_argument("null out");
     *out = nullptr;
     auto st = std::make_unique<casl_stencil>();
-    upload_tables(tab, st.get());
+    upload_tables(tab, st.get(), true);
     *out = st.release();
   });
 }
@@ -343,6 +472,27 @@ int casl_stencil_create(const casl_flat_stencil *tab, casl_stencil_t **out) {
 int casl_stencil_destroy(casl_stencil_t *st) {
   delete st;
   return CASL_OK;
+}
+
+int casl_stencil_set_path(casl_stencil_t *st, int path) {
+  return guarded([&]() {
+    if (!st) throw std::invalid_argument("null stencil");
+    if (path != CASL_PATH_TILED && path != CASL_PATH_TWO_PASS) throw std::invalid_argument("unknown path");
+    st->path = path;
+  });
+}
+
+int casl_stencil_tile_info(const casl_stencil_t *st, int32_t *tiled, int64_t *n_tiles, int64_t *n_halo,
+                           int64_t *n_records, int64_t *n_irregular) {
+  return guarded([&]() {
+    if (!st) throw std::invalid_argument("null stencil");
+    if (tiled) *tiled = st->tiled ? 1 : 0;
+    if (n_tiles) *n_tiles = st->n_tiles;
+    if (n_halo) *n_halo = st->n_halo;
+    if (n_records) *n_records = st->n_records;
+    if (n_irregular) *n_irregular = st->n_irregular;
+    if (!st->tiled && !st->tile_error.empty()) set_error(st->tile_error);
+  });
 }
 
 int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes) {

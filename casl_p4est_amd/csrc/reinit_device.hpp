@@ -167,41 +167,54 @@ __device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, const d
 }
 
 // Pass B at one node: the stage result of reinitialize_one_iteration
-// (src/my_p4est_level_set.cpp:82-281) given the prepared flags / s_fix.
-template <int D, class R, bool SECOND_ORDER>
-__device__ __forceinline__ R node_stage(const Tables<D> &t, int64_t n, uint16_t fl, const R *__restrict__ s_fix,
-                                        const double *__restrict__ phi0, const R *__restrict__ phin,
-                                        const R *__restrict__ dd) {
+// (src/my_p4est_level_set.cpp:82-281) given the prepared flags / arm lengths.
+//   t, j      stencil accessor and the node's index in it and in phin / dd
+//   fl, w     flags and wall bits of the node
+//   sfix_n    s_fix + n (entries a stride sfix_stride apart, one per face)
+//   phi0_n    phi_0 + n (read only for nodes outside the limits)
+//   dd        pass-A output, component a at dd[a * dd_stride + index]
+//   OWN_DD    the node's pass-A values are taken from dd[a * dd_stride + j]
+//             (they are the laplace of this node by the same arithmetic)
+template <int D, class R, bool SECOND_ORDER, class TB, bool OWN_DD = false>
+__device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, uint8_t w,
+                                         const R *__restrict__ sfix_n, int64_t sfix_stride,
+                                         const double *__restrict__ phi0_n, const R *__restrict__ phin,
+                                         const R *__restrict__ dd, int64_t dd_stride) {
   constexpr int F = 2 * D;
   const int mode = fl & 3;
   if (mode == MODE_ANCHOR) return R(0.0);
-  if (mode == MODE_COPY0) return R(phi0[n]);
-  if (mode == MODE_MASKED) return phin[n];
+  if (mode == MODE_COPY0) return R(*phi0_n);
+  if (mode == MODE_MASKED) return phin[j];
   R pn_000, pn_f[F], pn_dd_000[D];
   // ngbd_with_quadratic_interpolation of phi_n; its laplace is the node's own
-  // pass-A value (same arithmetic), so phi_n_xx_000 comes from here.
-  ngbd_quadratic<D, R>(t, n, phin, pn_000, pn_f, pn_dd_000);
+  // pass-A value (same arithmetic), so phi_n_xx_000 comes from there.
+  if constexpr (OWN_DD && SECOND_ORDER) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) pn_dd_000[a] = dd[int64_t(a) * dd_stride + j];
+    ngbd_quadratic_given_dd<D, R>(t, j, phin, pn_dd_000, pn_000, pn_f);
+  } else {
+    ngbd_quadratic<D, R>(t, j, phin, pn_000, pn_f, pn_dd_000);
+  }
   R pn_dd_f[F];
   if constexpr (SECOND_ORDER) {
 #pragma unroll
-    for (int f = 0; f < F; ++f) pn_dd_f[f] = face_linear<D, R>(t, f, n, dd + int64_t(f >> 1) * t.N);
+    for (int f = 0; f < F; ++f) pn_dd_f[f] = face_linear<D, R>(t, f, j, dd + int64_t(f >> 1) * dd_stride);
   }
   R s[F];
 #pragma unroll
-  for (int f = 0; f < F; ++f) s[f] = R(t.d(f, n));
+  for (int f = 0; f < F; ++f) s[f] = R(t.d(f, j));
   const unsigned cross = (fl >> 3) & ((1u << F) - 1);
   if (cross) {
 #pragma unroll
     for (int f = 0; f < F; ++f)
       if (cross & (1u << f)) {
-        s[f] = s_fix[int64_t(f) * t.N + n];
+        s[f] = sfix_n[int64_t(f) * sfix_stride];
         pn_f[f] = R(0.0);
       }
     // MAX(s, zero_distance_threshold) of the non-crossing arms only changes
     // negative wall arms, which the Neumann mirror below overwrites.
   }
   // Neumann boundary condition on the walls (:216-225)
-  const uint8_t w = t.wall[n];
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     const int fm = 2 * a, fp = 2 * a + 1;
@@ -251,6 +264,14 @@ __device__ __forceinline__ R node_stage(const Tables<D> &t, int64_t n, uint16_t 
   return r;
 }
 
+// stage_value on the grid-wide tables
+template <int D, class R, bool SECOND_ORDER>
+__device__ __forceinline__ R node_stage(const Tables<D> &t, int64_t n, uint16_t fl, const R *__restrict__ s_fix,
+                                        const double *__restrict__ phi0, const R *__restrict__ phin,
+                                        const R *__restrict__ dd) {
+  return stage_value<D, R, SECOND_ORDER>(t, n, fl, t.wall[n], s_fix + n, t.N, phi0 + n, phin, dd, t.N);
+}
+
 }  // namespace dev
 }  // namespace casl
 
@@ -268,6 +289,16 @@ struct casl_stencil {
   casl::DevBuf<float> phif, tmpf, ddnf, sfixf;
   casl::DevBuf<uint16_t> flags;
   casl::DevBuf<unsigned long long> red;
+  // tiled encoding (tiles.hpp)
+  bool tiled = false;
+  int path = CASL_PATH_TILED;
+  int64_t n_tiles = 0, n_halo = 0, n_records = 0, n_irregular = 0;
+  double geom_base[3] = {0, 0, 0}, geom_unit[3] = {0, 0, 0};
+  std::string tile_error;
+  casl::DevBuf<int32_t> t_tile_start, t_halo_start, t_h1count, t_halo, t_rec_start;
+  casl::DevBuf<uint16_t> t_rec_slot;
+  casl::DevBuf<uint4> t_rec;
+  casl::DevBuf<uint16_t> t_ext;
 
   template <int D>
   casl::dev::Tables<D> tables() const {
@@ -282,7 +313,8 @@ struct casl_stencil {
 
 namespace casl {
 // Validates and uploads host tables into st (shared by stencils and batches).
-void upload_tables(const casl_flat_stencil *tab, casl_stencil *st);
+// with_tiles also builds the tiled encoding for the fused stage kernel.
+void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles);
 
 // Kernel timing (casl_profile_*): prof_begin/prof_end bracket one launch.
 bool prof_on();

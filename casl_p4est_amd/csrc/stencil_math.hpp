@@ -18,6 +18,7 @@ constexpr double EPS = 1e-13;  // src/casl_math.h:23
 // stored as FP32 (exact for dyadic trees; checked at upload).
 template <int D>
 struct Tables {
+  static constexpr bool REGULAR = false;  // see the REGULAR shortcuts below
   static constexpr int F = 2 * D;
   static constexpr int C = 1 << (D - 1);
   static constexpr int T = 2 * (D - 1);
@@ -74,8 +75,20 @@ __device__ __forceinline__ double interface_location_2nd(double a, double b, dou
 
 // get_linear_interpolation_weights + interpolate_linearly,
 // my_p4est_quad_neighbor_nodes_of_node.h:222-306
-template <int D, class R, class G>
-__device__ __forceinline__ R face_linear(const Tables<D> &t, int f, int64_t n, const G *__restrict__ g) {
+// TB is any stencil accessor with idx(f, c, n), d(f, n) and dt(f, k, n):
+// Tables<D> (global SoA tables) or a decoded per-node stencil (LDS-local
+// indices, see tiles_device.hpp).
+//
+// TB::REGULAR marks a node none of whose arms hangs on a coarser cell: every
+// face then has exactly one corner c with a nonzero weight w, the reference's
+// (sum_c f_c w_c)/sum-of-weights is (f_c w)/w = f_c exactly (w and the
+// normalisation are equal powers of two), every m_ij of
+// correct_naive_second_derivatives is 0 and every quadratic correction is
+// 0.5*0*f'' = 0.  The REGULAR shortcuts therefore return the same values as
+// the full expressions for finite inputs (at most the sign of a zero differs).
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G *__restrict__ g) {
+  if constexpr (TB::REGULAR) return R(g[t.idx(f, 0, n)]);
   const R d_m0 = R(t.dt(f, 0, n)), d_p0 = R(t.dt(f, 1, n));
   if constexpr (D == 2) {
     const R w_mm = d_p0;
@@ -105,8 +118,9 @@ template <class R>
 __device__ __forceinline__ bool check_if_zero(R v) { return fabs(v) < R(EPS); }
 
 // correct_naive_second_derivatives, my_p4est_quad_neighbor_nodes_of_node.cpp:44-298
-template <int D, class R>
-__device__ __forceinline__ void correct_naive(const Tables<D> &t, int64_t n, R &Dxx, R &Dyy, R &Dzz) {
+template <int D, class R, class TB>
+__device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R &Dyy, R &Dzz) {
+  if constexpr (TB::REGULAR) return;
   const R d_m00 = R(t.d(0, n)), d_p00 = R(t.d(1, n)), d_0m0 = R(t.d(2, n)), d_0p0 = R(t.d(3, n));
   const R m12 = R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)) / d_m00 / (d_p00 + d_m00) +
                 R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)) / d_p00 / (d_p00 + d_m00);
@@ -202,9 +216,8 @@ __device__ __forceinline__ void correct_naive(const Tables<D> &t, int64_t n, R &
 }
 
 // laplace, my_p4est_quad_neighbor_nodes_of_node.h:615-635 (one field)
-template <int D, class R, class G>
-__device__ __forceinline__ void laplace(const Tables<D> &t, int64_t n, const G *__restrict__ g, R &f000, R *fnb,
-                                        R *dd) {
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ void laplace(const TB &t, int64_t n, const G *__restrict__ g, R &f000, R *fnb, R *dd) {
   f000 = R(g[n]);
 #pragma unroll
   for (int f = 0; f < 2 * D; ++f) fnb[f] = face_linear<D, R>(t, f, n, g);
@@ -216,11 +229,11 @@ __device__ __forceinline__ void laplace(const Tables<D> &t, int64_t n, const G *
   if constexpr (D == 3) dd[2] = dzz;
 }
 
-// ngbd_with_quadratic_interpolation, my_p4est_quad_neighbor_nodes_of_node.h:1514-1545
-template <int D, class R, class G>
-__device__ __forceinline__ void ngbd_quadratic(const Tables<D> &t, int64_t n, const G *__restrict__ g, R &f000,
-                                               R *fnb, R *dd) {
-  laplace<D, R>(t, n, g, f000, fnb, dd);
+// The third-order corrections of ngbd_with_quadratic_interpolation,
+// my_p4est_quad_neighbor_nodes_of_node.h:1530-1538: fnb[f] -= 0.5 d_m0 d_p0 f_tt (+ ...)
+template <int D, class R, class TB>
+__device__ __forceinline__ void quadratic_corrections(const TB &t, int64_t n, R *fnb, const R *dd) {
+  if constexpr (TB::REGULAR) return;
   const R h = R(0.5);
   if constexpr (D == 2) {
     fnb[0] -= (h * R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)) * dd[1]);
@@ -235,6 +248,25 @@ __device__ __forceinline__ void ngbd_quadratic(const Tables<D> &t, int64_t n, co
     fnb[4] -= (h * R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)) * dd[0] + h * R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)) * dd[1]);
     fnb[5] -= (h * R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)) * dd[0] + h * R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)) * dd[1]);
   }
+}
+
+// ngbd_with_quadratic_interpolation, my_p4est_quad_neighbor_nodes_of_node.h:1514-1545
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ void ngbd_quadratic(const TB &t, int64_t n, const G *__restrict__ g, R &f000, R *fnb,
+                                               R *dd) {
+  laplace<D, R>(t, n, g, f000, fnb, dd);
+  quadratic_corrections<D, R>(t, n, fnb, dd);
+}
+
+// The same with the node's own second derivatives already known (they are the
+// laplace values of this node, computed by the same arithmetic elsewhere).
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ void ngbd_quadratic_given_dd(const TB &t, int64_t n, const G *__restrict__ g,
+                                                        const R *dd, R &f000, R *fnb) {
+  f000 = R(g[n]);
+#pragma unroll
+  for (int f = 0; f < 2 * D; ++f) fnb[f] = face_linear<D, R>(t, f, n, g);
+  quadratic_corrections<D, R>(t, n, fnb, dd);
 }
 
 }  // namespace dev

@@ -1,0 +1,86 @@
+// Tiled stencil encoding for the fused stage kernel (host and device view).
+//
+// Nodes are cut into tiles of consecutive Morton indices.  For a tile T the
+// host precomputes
+//   H1 = nodes outside T that carry a nonzero interpolation weight in the
+//        stencil of a node of T (their second derivatives are needed),
+//   H2 = nodes outside T u H1 in the stencils of H1 nodes,
+// and the local numbering  T (n - tile_start) | H1 | H2  used for the
+// tile's LDS arrays.  Every node of T u H1 gets a 16-byte record:
+//   idx[f]     LDS-local index of the one corner with nonzero weight on
+//              face f (a "regular" node: no arm hangs on a coarser cell)
+//   geo        4-bit level of each face's arm cell, wall bits, irregular bit
+// A node with a hanging arm ("irregular") stores a 32-bit index into an
+// extension table instead, whose per-face entry holds all corner indices and
+// the transverse offsets (in units of the finest cell of the tree).
+// Within a tile the records are grouped (T regular, T hanging, H1 regular,
+// H1 hanging) so wavefronts rarely mix the two code paths; rec_slot gives
+// each record's LDS slot.
+// Decoding reproduces the flattened stencil of the node exactly (zero-weight
+// corners aside, which contribute exactly 0 for finite values); the builder
+// verifies this for every record.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/casl_reinit.h"
+
+namespace casl {
+
+struct alignas(16) TileRec {
+  uint16_t idx[6];
+  uint32_t geo;  // bits 0-23: face levels (4 bits each), 24-29: wall bits, 30: irregular
+};
+static_assert(sizeof(TileRec) == 16, "TileRec must be 16 bytes");
+
+constexpr uint32_t GEO_IRREGULAR = 1u << 30;
+
+template <int D>
+struct FaceExt;
+template <>
+struct FaceExt<2> {
+  uint16_t c[2];
+  uint16_t m0;  // d_m0 in units of the finest cell along the transverse axis
+};
+template <>
+struct FaceExt<3> {
+  uint16_t c[4];
+  uint16_t m0;  // along the first transverse axis
+  uint16_t zm;  // d_0m along the second transverse axis
+};
+
+// Tile sizing: at most TMAX tile nodes, LMAX nodes in T u H1 u H2 (LDS phi
+// slots) and DMAX nodes in T u H1 (LDS second-derivative slots).
+template <int D>
+struct TileLimits;
+template <>
+struct TileLimits<2> {
+  static constexpr int TMAX = 1024, LMAX = 2048, DMAX = 1536;
+};
+template <>
+struct TileLimits<3> {
+  static constexpr int TMAX = 512, LMAX = 2048, DMAX = 1024;
+};
+
+struct TileHost {
+  int dim = 0;
+  int lmax = 0;             // finest arm level present
+  double size[3] = {1, 1, 1};  // tree size per axis (box edge)
+  std::vector<int32_t> tile_start;  // [ntiles + 1] node ranges
+  std::vector<int32_t> halo_start;  // [ntiles + 1] ranges in halo_idx
+  std::vector<int32_t> h1count;     // [ntiles] number of H1 entries (first in each halo range)
+  std::vector<int32_t> halo_idx;    // global node ids of H1 then H2, per tile
+  std::vector<int32_t> rec_start;   // [ntiles + 1] ranges in rec (T u H1 of each tile)
+  std::vector<TileRec> rec;         // per tile: T regular, T hanging, H1 regular, H1 hanging
+  std::vector<uint16_t> rec_slot;   // LDS slot (local index) of each record
+  std::vector<uint16_t> ext;        // irregular extensions, FaceExt<D>[2D] each, as raw uint16
+  int64_t n_irregular = 0;
+};
+
+// Builds the tiled encoding of a flattened stencil; throws std::runtime_error
+// when the stencil cannot be encoded exactly (then the grid-wide kernels are
+// used instead).  box sizes are recovered from the arm lengths.
+void build_tiles(const casl_flat_stencil &tab, TileHost &out);
+
+}  // namespace casl

@@ -1,0 +1,171 @@
+// Device side of the tiled stencil encoding (tiles.hpp): record decoding and
+// the fused pseudo-time stage kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "reinit_device.hpp"
+#include "tiles.hpp"
+
+namespace casl {
+namespace dev {
+
+template <class R>
+struct TileGeom {
+  R base[3];  // coarsest arm-cell size per axis
+  R unit[3];  // finest cell size per axis
+};
+
+__device__ __forceinline__ double pow2neg(double, int l) { return __longlong_as_double((1023ll - l) << 52); }
+__device__ __forceinline__ float pow2neg(float, int l) { return __int_as_float((127 - l) << 23); }
+
+// Face distances of a record: arm-cell sizes from the level nibbles, wall arms
+// -(d_opposite) - (second-degree cell size) (construct_neighbors :609-719).
+template <int D, class R>
+__device__ __forceinline__ void decode_dist(uint32_t geo, const TileGeom<R> &g, R *dist, R *q) {
+  constexpr int F = 2 * D;
+#pragma unroll
+  for (int f = 0; f < F; ++f) q[f] = g.base[f >> 1] * pow2neg(R(0), int((geo >> (4 * f)) & 15u));
+#pragma unroll
+  for (int f = 0; f < F; ++f) dist[f] = q[f];
+#pragma unroll
+  for (int f = 0; f < F; ++f)
+    if ((geo >> (24 + f)) & 1u) dist[f] = -dist[f ^ 1] - q[f];
+}
+
+// Stencil of a regular node (no hanging arm): one LDS-local corner per face.
+template <int D, class R>
+struct RegStencil {
+  static constexpr bool REGULAR = true;
+  static constexpr int F = 2 * D;
+  uint16_t id[F];
+  R dist[F];
+  R q[F];
+  __device__ __forceinline__ int32_t idx(int f, int, int64_t) const { return id[f]; }
+  __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
+  // transverse offsets (0, arm cell) -- only their product (0) enters the math
+  __device__ __forceinline__ R dt(int f, int k, int64_t) const { return (k & 1) ? q[f] : R(0); }
+  __device__ __forceinline__ void load(const TileRec &r, const TileGeom<R> &g) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) id[f] = r.idx[f];
+    decode_dist<D, R>(r.geo, g, dist, q);
+  }
+};
+
+// Stencil of a node with a hanging arm: corners and transverse offsets are
+// read from its extension record when the math asks for them.
+template <int D, class R>
+struct IrrStencil {
+  static constexpr bool REGULAR = false;
+  static constexpr int F = 2 * D, C = 1 << (D - 1);
+  static constexpr int W = (D == 2 ? 3 : 6);  // uint16 words per face extension
+  const uint16_t *__restrict__ e;
+  R dist[F];
+  R q[F];
+  R unit;
+  __device__ __forceinline__ int32_t idx(int f, int c, int64_t) const { return e[f * W + c]; }
+  __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
+  __device__ __forceinline__ R dt(int f, int k, int64_t) const {
+    const R o = R(e[f * W + C + (k >> 1)]) * unit;  // m0 (k = 0, 1) or 0m (k = 2, 3)
+    return (k & 1) ? q[f] - o : o;
+  }
+  __device__ __forceinline__ void load(const TileRec &r, const uint16_t *__restrict__ ext, const TileGeom<R> &g) {
+    const uint32_t id = uint32_t(r.idx[0]) | (uint32_t(r.idx[1]) << 16);
+    e = ext + size_t(id) * (F * W);
+    unit = g.unit[0];
+    decode_dist<D, R>(r.geo, g, dist, q);
+  }
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; give every XCD one
+// contiguous run of tiles so neighbouring tiles (which share halo nodes)
+// meet in the same L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  constexpr int X = 8;
+  const int q = nb / X, r = nb % X;
+  const int x = b % X, k = b / X;
+  return x * q + (x < r ? x : r) + k;
+}
+
+template <class R>
+struct TileArgs {
+  const int32_t *__restrict__ tile_start;
+  const int32_t *__restrict__ halo_start;
+  const int32_t *__restrict__ h1count;
+  const int32_t *__restrict__ halo;
+  const int32_t *__restrict__ rec_start;
+  const TileRec *__restrict__ rec;
+  const uint16_t *__restrict__ rec_slot;
+  const uint16_t *__restrict__ ext;
+  TileGeom<R> geom;
+  const uint16_t *__restrict__ flags;
+  const R *__restrict__ sfix;
+  const double *__restrict__ phi0;
+  const R *__restrict__ phin;
+  R *out;
+  int64_t N;
+};
+
+constexpr int TILE_BLOCK = 256;
+
+// One pseudo-time stage of a tile: phi of T u H1 u H2 into LDS, pass A for
+// T u H1 in LDS, pass B for T.  out = result, or (RK_AVG) out = (out + result)/2.
+template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
+__global__ void __launch_bounds__(TILE_BLOCK) k_tile_stage(TileArgs<R> A) {
+  constexpr int LMAX = TileLimits<D>::LMAX, DMAX = TileLimits<D>::DMAX;
+  __shared__ R phis[LMAX];
+  __shared__ R dds[SECOND_ORDER ? D * DMAX : 1];
+  const int tile = xcd_tile(int(blockIdx.x), int(gridDim.x));
+  const int s = A.tile_start[tile], nT = A.tile_start[tile + 1] - s;
+  const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
+  const int nH1 = A.h1count[tile];
+  const int rb = A.rec_start[tile];
+  for (int j = threadIdx.x; j < nT; j += TILE_BLOCK) phis[j] = A.phin[s + j];
+  for (int j = threadIdx.x; j < nH; j += TILE_BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
+  __syncthreads();
+  if constexpr (SECOND_ORDER) {
+    for (int i = threadIdx.x; i < nT + nH1; i += TILE_BLOCK) {
+      const TileRec rec = A.rec[rb + i];
+      const int j = A.rec_slot[rb + i];
+      R f000, fnb[2 * D], dd[D];
+      if (!(rec.geo & GEO_IRREGULAR)) {
+        RegStencil<D, R> st;
+        st.load(rec, A.geom);
+        laplace<D, R>(st, j, phis, f000, fnb, dd);
+      } else {
+        IrrStencil<D, R> st;
+        st.load(rec, A.ext, A.geom);
+        laplace<D, R>(st, j, phis, f000, fnb, dd);
+      }
+#pragma unroll
+      for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < nT; i += TILE_BLOCK) {
+    const TileRec rec = A.rec[rb + i];
+    const int j = A.rec_slot[rb + i];
+    const int64_t n = int64_t(s) + j;
+    const uint16_t fl = A.flags[n];
+    const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
+    R r;
+    if (!(rec.geo & GEO_IRREGULAR)) {
+      RegStencil<D, R> st;
+      st.load(rec, A.geom);
+      r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + n, A.N, A.phi0 + n, phis,
+                                                                   dds, DMAX);
+    } else {
+      IrrStencil<D, R> st;
+      st.load(rec, A.ext, A.geom);
+      r = stage_value<D, R, SECOND_ORDER, IrrStencil<D, R>, true>(st, j, fl, w, A.sfix + n, A.N, A.phi0 + n, phis,
+                                                                   dds, DMAX);
+    }
+    if constexpr (RK_AVG)
+      A.out[n] = R(0.5) * (A.out[n] + r);
+    else
+      A.out[n] = r;
+  }
+}
+
+}  // namespace dev
+}  // namespace casl

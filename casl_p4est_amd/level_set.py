@@ -55,6 +55,23 @@ class DeviceStencil:
     def handle(self):
         return self._h
 
+    def set_path(self, path):
+        """"tiled" (fused stage kernel over LDS-staged Morton tiles, default when
+        the stencil could be tiled) or "two_pass" (grid-wide pass A + pass B)."""
+        code = {"tiled": L.CASL_PATH_TILED, "two_pass": L.CASL_PATH_TWO_PASS}[path]
+        L.call("casl_stencil_set_path", self._h, code)
+
+    def tile_info(self):
+        vals = [ctypes.c_int32()] + [ctypes.c_int64() for _ in range(4)]
+        L.call("casl_stencil_tile_info", self._h, *[ctypes.byref(v) for v in vals])
+        keys = ("tiled", "n_tiles", "n_halo", "n_records", "n_irregular")
+        out = {k: v.value for k, v in zip(keys, vals)}
+        out["tiled"] = bool(out["tiled"])
+        if not out["tiled"]:
+            msg = L.lib().casl_last_error()
+            out["reason"] = msg.decode() if msg else ""
+        return out
+
     def _check_field(self, t, name):
         torch = _torch()
         if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.float64:

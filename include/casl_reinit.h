@@ -91,6 +91,19 @@ int casl_stencil_create(const casl_flat_stencil *tables, casl_stencil_t **out);
 int casl_stencil_destroy(casl_stencil_t *st);
 int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes);
 
+/* Stage kernel selection.  CASL_PATH_TILED (default when the stencil could
+ * be tiled): one fused kernel per pseudo-time stage over Morton tiles with
+ * LDS-staged halos; CASL_PATH_TWO_PASS: grid-wide pass A + pass B kernels. */
+#define CASL_PATH_TILED    0
+#define CASL_PATH_TWO_PASS 1
+int casl_stencil_set_path(casl_stencil_t *st, int path);
+/* tiled: 1 if the tiled encoding exists; counts of tiles, halo entries
+ * (H1 + H2 over all tiles), records (T + H1 over all tiles) and records of
+ * nodes with a hanging arm. */
+int casl_stencil_tile_info(const casl_stencil_t *st, int32_t *tiled,
+                           int64_t *n_tiles, int64_t *n_halo,
+                           int64_t *n_records, int64_t *n_irregular);
+
 /* second_derivatives_central: fdd[a*N + n] = d^2 f / dx_a^2 at node n.
  * f and fdd are FP64 device arrays. */
 int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd,

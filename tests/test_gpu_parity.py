@@ -54,15 +54,27 @@ def test_second_derivatives_match_oracle(cuda, name):
 
 
 @pytest.mark.parametrize("name", sorted(PROBLEMS))
+def test_tiled_encoding_built(cuda, name):
+    dim, shape, lmin, lmax = PROBLEMS[name]
+    f, ngbd, phi0 = problem(dim, shape, lmin, lmax)
+    info = ngbd.device().tile_info()
+    say(f"[gpu] tile info {name}: {info}")
+    assert info["tiled"], info
+    assert info["n_records"] >= ngbd.num_nodes and info["n_irregular"] > 0
+
+
+@pytest.mark.parametrize("name", sorted(PROBLEMS))
 @pytest.mark.parametrize("orders", [(2, 2), (2, 1), (1, 2), (1, 1)])
-def test_reinit_fp64_matches_oracle(cuda, name, orders):
+@pytest.mark.parametrize("path", ["tiled", "two_pass"])
+def test_reinit_fp64_matches_oracle(cuda, name, orders, path):
     import torch
     dim, shape, lmin, lmax = PROBLEMS[name]
-    say(f"[gpu] reinit fp64 {name} orders={orders}")
+    say(f"[gpu] reinit fp64 {name} orders={orders} path={path}")
     f, ngbd, phi0 = problem(dim, shape, lmin, lmax)
     it = 12
     ref = O.reinit(ngbd.tables, phi0, orders[0], orders[1], it)
     ls_ = C.LevelSet(ngbd)
+    ngbd.device().set_path(path)
     phi = torch.from_numpy(phi0).to(cuda)
     ls_.reinitialize_within_range_of_phi_0(phi, orders[0], orders[1], C.DBL_MAX, -C.DBL_MAX, it)
     out = phi.cpu().numpy()
@@ -70,10 +82,12 @@ def test_reinit_fp64_matches_oracle(cuda, name, orders):
     say(f"    max |gpu - oracle| = {np.max(np.abs(out - ref)):.3e}  identical={np.array_equal(out, ref)}")
 
 
-def test_reinit_limits_and_mask(cuda):
+@pytest.mark.parametrize("path", ["tiled", "two_pass"])
+def test_reinit_limits_and_mask(cuda, path):
     import torch
     dim, shape, lmin, lmax = PROBLEMS["star2d"]
     f, ngbd, phi0 = problem(dim, shape, lmin, lmax)
+    ngbd.device().set_path(path)
     _, xyz = f.nodes()
     lsr = C.LevelSet(ngbd)
     say("[gpu] limits")
@@ -102,12 +116,14 @@ def test_invalid_orders_raise(cuda):
 
 
 @pytest.mark.parametrize("name", ["star2d.npz", "spheres3d.npz"])
-def test_golden_fixtures(cuda, name):
+@pytest.mark.parametrize("path", ["tiled", "two_pass"])
+def test_golden_fixtures(cuda, name, path):
     import torch
-    say(f"[gpu] golden {name}")
+    say(f"[gpu] golden {name} {path}")
     z = np.load(os.path.join(G, name), allow_pickle=False)
     tab = C.FlatStencil(int(z["dim"]), z["nbr"], z["d"], z["d_tr"], z["wall"], float(z["zdt"]))
     dev = C.DeviceStencil(tab)
+    dev.set_path(path)
     phi0 = z["phi0"]
     g = torch.from_numpy(phi0).to(cuda)
     assert np.array_equal(dev.second_derivatives(g).cpu().numpy(), z["dd0"])
@@ -135,11 +151,13 @@ def test_config1_star_fp64_matches_oracle(cuda):
 
 
 @pytest.mark.parametrize("name", sorted(PROBLEMS))
-def test_mixed_precision_error_band(cuda, name):
+@pytest.mark.parametrize("path", ["tiled", "two_pass"])
+def test_mixed_precision_error_band(cuda, name, path):
     import torch
     dim, shape, lmin, lmax = PROBLEMS[name]
-    say(f"[gpu] mixed vs fp64 {name}")
+    say(f"[gpu] mixed vs fp64 {name} {path}")
     f, ngbd, phi0 = problem(dim, shape, lmin, lmax)
+    ngbd.device().set_path(path)
     h = float(np.min(f.dxyz_min()))
     lsr = C.LevelSet(ngbd)
     a = torch.from_numpy(phi0).to(cuda)

@@ -102,6 +102,31 @@ SIGNATURES = {
                                          ctypes.c_void_p]),
     "casl_batch_reinit_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                                ctypes.c_void_p]),
+    "casl_curvature_samples": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_int,
+                                              ctypes.c_void_p, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                              ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                              c_int64_p, ctypes.c_void_p]),
+    "casl_mlp_create": (ctypes.c_int, [c_int32_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
+    "casl_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "casl_mlp_num_params": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p]),
+    "casl_mlp_set_scaler": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_float)]),
+    "casl_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_float)]),
+    "casl_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                           ctypes.POINTER(ctypes.c_float)]),
+    "casl_mlp_get_grads": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_float),
+                                          ctypes.POINTER(ctypes.c_float)]),
+    "casl_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                        ctypes.c_void_p]),
+    "casl_mlp_backward": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_float), ctypes.c_void_p]),
+    "casl_mlp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_float, ctypes.c_void_p]),
+    "casl_bf16_gemm": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                      ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                      ctypes.c_void_p]),
     "casl_levelset_eval": (ctypes.c_int, [ctypes.POINTER(LevelSetSpec), ctypes.c_int, ctypes.c_int64,
                                           c_double_p, c_double_p]),
     "casl_forest_new": (ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_void_p)]),

@@ -1,0 +1,412 @@
+// Small fully-connected ReLU network (h*kappa regressor) trained and run on
+// the GPU with hand-written BF16 MFMA kernels (FP32 accumulation).
+//
+// The reference runs a comparable network for inference only, on the CPU
+// with OpenBLAS sgemm (src/my_p4est_curvature_ml.cpp:214-286), and ships no
+// training code; this is the MI355X-native counterpart with training.
+//
+// Layer l: Z_l = A_{l-1} W_l + b_l, A_l = relu(Z_l) (last layer: no ReLU);
+// A_0 = bf16((x - mean)/std).  Weights are FP32 masters [in][out] with BF16
+// copies for the GEMMs; activations are stored in BF16; every GEMM
+// accumulates in FP32.  Loss: mean squared error.
+//
+// GEMM: one wavefront computes a 32x32 tile with
+// v_mfma_f32_32x32x16_bf16 (A lanes hold rows i = lane%32, k = 8*(lane/32)+v;
+// B lanes hold columns j = lane%32, same k; the 16 accumulators of a lane are
+// rows 8*(r/4) + 4*(lane/32) + r%4 of column lane%32); a 256-thread block
+// covers 64x64.  Every operand load is bounds checked, so any M, N, K work.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <vector>
+
+#include "reinit_device.hpp"
+
+using namespace casl;
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return 0x7fc0;  // NaN
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+__host__ __device__ inline float bf2f(uint16_t h) {
+  uint32_t u = uint32_t(h) << 16;
+  float f;
+  memcpy(&f, &u, 4);
+  return f;
+}
+
+// C[M x N] = op(A)[M x K] op(B)[K x N] (+ bias[N]) then ReLU or ReLU'-mask.
+//   op(A)(i, k) = TA ? A[k*lda + i] : A[i*lda + k]
+//   op(B)(k, j) = TB ? B[j*ldb + k] : B[k*ldb + j]
+//   mask: multiply by (mask[i*ldm + j] > 0), the ReLU derivative of a stored activation
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256)
+    k_gemm(int M, int N, int K, const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, int ldb,
+           const float *__restrict__ bias, const uint16_t *__restrict__ mask, int ldm, int relu,
+           float *__restrict__ C32, uint16_t *__restrict__ C16, int ldc) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int tm = blockIdx.y * 64 + (wave >> 1) * 32;
+  const int tn = blockIdx.x * 64 + (wave & 1) * 32;
+  const int li = lane & 31, lk = (lane >> 5) * 8;
+  const int i = tm + li, j = tn + li;
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  for (int k0 = 0; k0 < K; k0 += 16) {
+    bf16x8 a, b;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int k = k0 + lk + v;
+      uint16_t av = 0, bv = 0;
+      if (i < M && k < K) av = TA ? A[size_t(k) * lda + i] : A[size_t(i) * lda + k];
+      if (j < N && k < K) bv = TB ? B[size_t(j) * ldb + k] : B[size_t(k) * ldb + j];
+      a[v] = __builtin_bit_cast(__bf16, av);
+      b[v] = __builtin_bit_cast(__bf16, bv);
+    }
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = tm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int col = tn + li;
+    if (row < M && col < N) {
+      float v = acc[r];
+      if (bias) v += bias[col];
+      if (relu) v = v > 0.f ? v : 0.f;
+      if (mask && !(bf2f(mask[size_t(row) * ldm + col]) > 0.f)) v = 0.f;
+      if (C32) C32[size_t(row) * ldc + col] = v;
+      if (C16) C16[size_t(row) * ldc + col] = f2bf(v);
+    }
+  }
+}
+
+// A_0 = bf16((x - mean) / std)
+__global__ void k_pack_input(const float *__restrict__ x, const float *__restrict__ mean,
+                             const float *__restrict__ inv_std, int n, int d, uint16_t *__restrict__ out) {
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= int64_t(n) * d) return;
+  const int c = int(t % d);
+  out[t] = f2bf((x[t] - mean[c]) * inv_std[c]);
+}
+
+// dY = 2 (y - t) / n (MSE gradient), as FP32 and BF16; loss accumulates sum (y - t)^2
+__global__ void k_mse_seed(const float *__restrict__ y, const float *__restrict__ t, int n, float scale,
+                           float *__restrict__ dy32, uint16_t *__restrict__ dy16, float *__restrict__ loss) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  float e = 0.f;
+  if (i < n) {
+    e = y[i] - t[i];
+    const float g = 2.f * e * scale;
+    dy32[i] = g;
+    dy16[i] = f2bf(g);
+  }
+  float s = e * e;
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
+}
+
+// column sums of an [n x m] FP32 matrix (bias gradients)
+__global__ void k_colsum(const float *__restrict__ x, int n, int m, float *__restrict__ out) {
+  __shared__ float red[256];
+  const int c = blockIdx.x;
+  float s = 0.f;
+  for (int r = threadIdx.x; r < n; r += blockDim.x) s += x[size_t(r) * m + c];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[c] = red[0];
+}
+
+// Adam on FP32 masters, refreshes the BF16 copy (may be null for biases)
+__global__ void k_adam(float *__restrict__ p, const float *__restrict__ g, float *__restrict__ m, float *__restrict__ v,
+                       uint16_t *__restrict__ p16, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gi;
+  const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float pi = p[i] - lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+  p[i] = pi;
+  if (p16) p16[i] = f2bf(pi);
+}
+
+__global__ void k_to_bf16(const float *__restrict__ a, uint16_t *__restrict__ b, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = f2bf(a[i]);
+}
+
+__global__ void k_zero_f(float *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.f;
+}
+
+inline unsigned blocks(int64_t n, int b = 256) { return unsigned((n + b - 1) / b); }
+
+template <bool TA, bool TB>
+void gemm(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, const float *bias,
+          const uint16_t *mask, int ldm, int relu, float *C32, uint16_t *C16, int ldc, hipStream_t s) {
+  dim3 grid(unsigned((N + 63) / 64), unsigned((M + 63) / 64));
+  prof_begin("k_gemm", s);
+  hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, mask, ldm, relu, C32,
+                     C16, ldc);
+  prof_end("k_gemm", s);
+  HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+struct casl_mlp {
+  int L = 0;                        // number of layers
+  std::vector<int> dim;             // L + 1 widths
+  DevBuf<float> W, b, gW, gb, mW, vW, mb, vb;  // concatenated per layer
+  DevBuf<uint16_t> W16;
+  std::vector<size_t> offW, offb;   // per-layer offsets
+  DevBuf<float> mean, inv_std;
+  int cap = 0;                      // activation capacity (rows)
+  DevBuf<uint16_t> act16;           // L activations [cap x dim[l]] (l = 0 .. L-1) as bf16
+  DevBuf<float> z32;                // FP32 scratch [cap x max width]
+  DevBuf<uint16_t> d16a, d16b;      // BF16 back-propagated gradients
+  DevBuf<float> y32, loss;
+  std::vector<size_t> offA;
+  int maxw = 0;
+  int64_t step = 0;
+
+  void ensure_cap(int n) {
+    if (n <= cap) return;
+    cap = n;
+    size_t tot = 0;
+    offA.assign(L, 0);
+    for (int l = 0; l < L; ++l) {
+      offA[l] = tot;
+      tot += size_t(n) * dim[l];
+    }
+    act16.ensure(tot);
+    z32.ensure(size_t(n) * maxw);
+    d16a.ensure(size_t(n) * maxw);
+    d16b.ensure(size_t(n) * maxw);
+    y32.ensure(size_t(n) * dim[L]);
+  }
+};
+
+namespace {
+
+// forward pass; leaves activations in act16 and the output in out (FP32 [n x dim[L]])
+void forward(casl_mlp *m, const float *x, int n, float *out, hipStream_t s) {
+  m->ensure_cap(n);
+  hipLaunchKernelGGL(k_pack_input, dim3(blocks(int64_t(n) * m->dim[0])), dim3(256), 0, s, x, m->mean.p,
+                     m->inv_std.p, n, m->dim[0], m->act16.p + m->offA[0]);
+  HIP_CHECK(hipGetLastError());
+  for (int l = 0; l < m->L; ++l) {
+    const int in = m->dim[l], o = m->dim[l + 1];
+    const uint16_t *A = m->act16.p + m->offA[l];
+    const bool last = (l == m->L - 1);
+    gemm<false, false>(n, o, in, A, in, m->W16.p + m->offW[l], o, m->b.p + m->offb[l], nullptr, 0, last ? 0 : 1,
+                       last ? out : nullptr, last ? nullptr : m->act16.p + m->offA[l + 1], o, s);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int casl_mlp_create(const int32_t *dims, int n_layers, uint64_t seed, casl_mlp_t **out) {
+  return guarded([&]() {
+    if (!out || !dims || n_layers < 1) throw std::invalid_argument("bad arguments");
+    auto m = std::make_unique<casl_mlp>();
+    m->L = n_layers;
+    m->dim.assign(dims, dims + n_layers + 1);
+    size_t nw = 0, nb = 0;
+    for (int l = 0; l < n_layers; ++l) {
+      if (m->dim[l] < 1 || m->dim[l + 1] < 1) throw std::invalid_argument("layer widths must be positive");
+      m->offW.push_back(nw);
+      m->offb.push_back(nb);
+      nw += size_t(m->dim[l]) * m->dim[l + 1];
+      nb += size_t(m->dim[l + 1]);
+    }
+    for (int d : m->dim) m->maxw = std::max(m->maxw, d);
+    // He-normal initialisation (deterministic in seed), zero biases
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<float> w(nw), bb(nb, 0.f);
+    for (int l = 0; l < n_layers; ++l) {
+      const double sd = std::sqrt(2.0 / m->dim[l]);
+      for (size_t i = 0; i < size_t(m->dim[l]) * m->dim[l + 1]; ++i) w[m->offW[l] + i] = float(sd * nd(rng));
+    }
+    std::vector<uint16_t> w16(nw);
+    for (size_t i = 0; i < nw; ++i) w16[i] = f2bf(w[i]);
+    m->W.ensure(nw);
+    m->W16.ensure(nw);
+    m->gW.ensure(nw);
+    m->mW.ensure(nw);
+    m->vW.ensure(nw);
+    m->b.ensure(nb);
+    m->gb.ensure(nb);
+    m->mb.ensure(nb);
+    m->vb.ensure(nb);
+    HIP_CHECK(hipMemcpy(m->W.p, w.data(), nw * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->W16.p, w16.data(), nw * 2, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->b.p, bb.data(), nb * 4, hipMemcpyHostToDevice));
+    std::vector<float> zw(nw, 0.f), zb(nb, 0.f);
+    HIP_CHECK(hipMemcpy(m->mW.p, zw.data(), nw * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->vW.p, zw.data(), nw * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->mb.p, zb.data(), nb * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->vb.p, zb.data(), nb * 4, hipMemcpyHostToDevice));
+    std::vector<float> mean(m->dim[0], 0.f), inv(m->dim[0], 1.f);
+    m->mean.ensure(mean.size());
+    m->inv_std.ensure(inv.size());
+    HIP_CHECK(hipMemcpy(m->mean.p, mean.data(), mean.size() * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->inv_std.p, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
+    m->loss.ensure(1);
+    *out = m.release();
+  });
+}
+
+int casl_mlp_destroy(casl_mlp_t *m) {
+  delete m;
+  return CASL_OK;
+}
+
+int casl_mlp_num_params(const casl_mlp_t *m, int64_t *n_weights, int64_t *n_biases) {
+  return guarded([&]() {
+    if (!m) throw std::invalid_argument("null mlp");
+    if (n_weights) *n_weights = int64_t(m->W.n);
+    if (n_biases) *n_biases = int64_t(m->b.n);
+  });
+}
+
+int casl_mlp_set_scaler(casl_mlp_t *m, const float *mean, const float *stdev) {
+  return guarded([&]() {
+    if (!m || !mean || !stdev) throw std::invalid_argument("null argument");
+    std::vector<float> inv(m->dim[0]);
+    for (int i = 0; i < m->dim[0]; ++i) {
+      if (!(stdev[i] > 0)) throw std::invalid_argument("scaler deviations must be positive");
+      inv[i] = 1.f / stdev[i];
+    }
+    HIP_CHECK(hipMemcpy(m->mean.p, mean, m->dim[0] * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->inv_std.p, inv.data(), m->dim[0] * 4, hipMemcpyHostToDevice));
+  });
+}
+
+int casl_mlp_get_params(const casl_mlp_t *m, float *weights, float *biases) {
+  return guarded([&]() {
+    if (!m) throw std::invalid_argument("null mlp");
+    if (weights) HIP_CHECK(hipMemcpy(weights, m->W.p, m->W.n * 4, hipMemcpyDeviceToHost));
+    if (biases) HIP_CHECK(hipMemcpy(biases, m->b.p, m->b.n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int casl_mlp_set_params(casl_mlp_t *m, const float *weights, const float *biases) {
+  return guarded([&]() {
+    if (!m || !weights || !biases) throw std::invalid_argument("null argument");
+    std::vector<uint16_t> w16(m->W.n);
+    for (size_t i = 0; i < m->W.n; ++i) w16[i] = f2bf(weights[i]);
+    HIP_CHECK(hipMemcpy(m->W.p, weights, m->W.n * 4, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->W16.p, w16.data(), m->W.n * 2, hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(m->b.p, biases, m->b.n * 4, hipMemcpyHostToDevice));
+  });
+}
+
+int casl_mlp_get_grads(const casl_mlp_t *m, float *gw, float *gb) {
+  return guarded([&]() {
+    if (!m) throw std::invalid_argument("null mlp");
+    if (gw) HIP_CHECK(hipMemcpy(gw, m->gW.p, m->gW.n * 4, hipMemcpyDeviceToHost));
+    if (gb) HIP_CHECK(hipMemcpy(gb, m->gb.p, m->gb.n * 4, hipMemcpyDeviceToHost));
+  });
+}
+
+int casl_mlp_forward(casl_mlp_t *m, const float *x, int n, float *y, void *stream) {
+  return guarded([&]() {
+    if (!m || !x || !y || n < 0) throw std::invalid_argument("bad arguments");
+    if (n == 0) return;
+    forward(m, x, n, y, static_cast<hipStream_t>(stream));
+  });
+}
+
+// Forward + backward of the mean squared error on (x, t); gradients stay on
+// the device (casl_mlp_get_grads).  *loss (host, may be null) receives the MSE.
+int casl_mlp_backward(casl_mlp_t *m, const float *x, const float *t, int n, float *loss, void *stream) {
+  return guarded([&]() {
+    if (!m || !x || !t || n <= 0) throw std::invalid_argument("bad arguments");
+    if (m->dim[m->L] != 1) throw std::invalid_argument("training needs a scalar output");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    forward(m, x, n, m->y32.p, s);
+    hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, s, m->loss.p, 1);
+    // dZ_L = 2 (y - t) / n  -> z32 (FP32) and d16a (BF16), [n x 1]
+    hipLaunchKernelGGL(k_mse_seed, dim3(blocks(n)), dim3(256), 0, s, m->y32.p, t, n, 1.f / float(n), m->z32.p,
+                       m->d16a.p, m->loss.p);
+    HIP_CHECK(hipGetLastError());
+    uint16_t *dcur = m->d16a.p, *dnext = m->d16b.p;
+    for (int l = m->L - 1; l >= 0; --l) {
+      const int in = m->dim[l], o = m->dim[l + 1];
+      // bias gradient from the FP32 dZ_l in z32
+      hipLaunchKernelGGL(k_colsum, dim3(o), dim3(256), 0, s, m->z32.p, n, o, m->gb.p + m->offb[l]);
+      HIP_CHECK(hipGetLastError());
+      const uint16_t *Aprev = m->act16.p + m->offA[l];  // [n x in]
+      // dW_l = A_{l-1}^T dZ_l : (in x n)(n x o)
+      gemm<true, false>(in, o, n, Aprev, in, dcur, o, nullptr, nullptr, 0, 0, m->gW.p + m->offW[l], nullptr, o, s);
+      if (l > 0) {
+        // dZ_{l-1} = (dZ_l W_l^T) * relu'(A_{l-1}) : (n x o)(o x in)
+        gemm<false, true>(n, in, o, dcur, o, m->W16.p + m->offW[l], o, nullptr, Aprev, in, 0, m->z32.p, dnext, in, s);
+        std::swap(dcur, dnext);
+      }
+    }
+    if (loss) {
+      float h = 0.f;
+      HIP_CHECK(hipMemcpyAsync(&h, m->loss.p, 4, hipMemcpyDeviceToHost, s));
+      HIP_CHECK(hipStreamSynchronize(s));
+      *loss = h / float(n);
+    }
+  });
+}
+
+int casl_mlp_adam_step(casl_mlp_t *m, float lr, float beta1, float beta2, float eps, void *stream) {
+  return guarded([&]() {
+    if (!m) throw std::invalid_argument("null mlp");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    m->step += 1;
+    const float c1 = float(1.0 / (1.0 - std::pow(double(beta1), double(m->step))));
+    const float c2 = float(1.0 / (1.0 - std::pow(double(beta2), double(m->step))));
+    const int nw = int(m->W.n), nb = int(m->b.n);
+    hipLaunchKernelGGL(k_adam, dim3(blocks(nw)), dim3(256), 0, s, m->W.p, m->gW.p, m->mW.p, m->vW.p, m->W16.p, nw,
+                       lr, beta1, beta2, eps, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(blocks(nb)), dim3(256), 0, s, m->b.p, m->gb.p, m->mb.p, m->vb.p,
+                       (uint16_t *)nullptr, nb, lr, beta1, beta2, eps, c1, c2);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+// Plain BF16 MFMA GEMM on device buffers (exposed for testing the kernel):
+// C = op(A) op(B) (+ bias) (relu), C32 and/or C16 written with leading dim ldc.
+int casl_bf16_gemm(int M, int N, int K, const uint16_t *A, int lda, int trans_a, const uint16_t *B, int ldb,
+                   int trans_b, const float *bias, int relu, float *C32, uint16_t *C16, int ldc, void *stream) {
+  return guarded([&]() {
+    if (M < 0 || N < 0 || K < 0 || !A || !B || (!C32 && !C16)) throw std::invalid_argument("bad arguments");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (!trans_a && !trans_b)
+      gemm<false, false>(M, N, K, A, lda, B, ldb, bias, nullptr, 0, relu, C32, C16, ldc, s);
+    else if (trans_a && !trans_b)
+      gemm<true, false>(M, N, K, A, lda, B, ldb, bias, nullptr, 0, relu, C32, C16, ldc, s);
+    else if (!trans_a && trans_b)
+      gemm<false, true>(M, N, K, A, lda, B, ldb, bias, nullptr, 0, relu, C32, C16, ldc, s);
+    else
+      throw std::invalid_argument("both operands transposed is not supported");
+  });
+}
+
+}  // extern "C"

@@ -110,8 +110,18 @@ constexpr int TILE_BLOCK = 256;
 
 // One pseudo-time stage of a tile: phi of T u H1 u H2 into LDS, pass A for
 // T u H1 in LDS, pass B for T.  out = result, or (RK_AVG) out = (out + result)/2.
+// Occupancy: the kernel is latency bound (gathers, two barriers per tile),
+// so the register budget is capped to keep TILE_WAVES_* waves per SIMD.
+#ifndef CASL_TILE_WAVES_F32
+#define CASL_TILE_WAVES_F32 4
+#endif
+#ifndef CASL_TILE_WAVES_F64
+#define CASL_TILE_WAVES_F64 3
+#endif
 template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
-__global__ void __launch_bounds__(TILE_BLOCK) k_tile_stage(TileArgs<R> A) {
+__global__ void __launch_bounds__(TILE_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
+    k_tile_stage(TileArgs<R> A) {
   constexpr int LMAX = TileLimits<D>::LMAX, DMAX = TileLimits<D>::DMAX;
   __shared__ R phis[LMAX];
   __shared__ R dds[SECOND_ORDER ? D * DMAX : 1];

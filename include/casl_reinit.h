@@ -167,6 +167,61 @@ int casl_batch_reinit(casl_batch_t *b, double *phi, int n_iter,
 int casl_batch_reinit_graph(casl_batch_t *b, double *phi, int n_iter,
                             int precision, void *stream);
 
+/* ------------------------------------------- learned curvature (h*kappa)
+ * The reference infers h*kappa with a CPU MLP (OpenBLAS sgemm) whose
+ * training happens outside the repository (src/my_p4est_curvature_ml.cpp:
+ * 214-286, 483-556).  Here the network is trained and evaluated on the GPU
+ * by BF16-MFMA kernels with FP32 accumulation.  All arrays below are device
+ * arrays unless marked host. */
+
+/* Numerical curvature samples on n_patch uniform (M+1)x(M+1) patches of
+ * spacing h with origin (x0, y0).  grid[i*(M+1)+j] is the patch-local node
+ * index of (x0 + i h, y0 + j h); phi holds the patches' node values one after
+ * another (n_nodes each).  For every node with an opposite-sign face
+ * neighbour and 4 <= i, j <= M-4 one sample is appended (order unspecified):
+ * feat[s][0..8] = 3x3 phi stencil / h (x slowest), feat[s][9] = h*kappa
+ * bilinearly interpolated at the projection x - phi n (kappa = div n, central
+ * differences: compute_normals + qnnn::get_curvature), proj[s] = that
+ * projection, node[s] = p*n_nodes + node index.  *n_samples (host) = number
+ * found; only the first max_samples are written. */
+int casl_curvature_samples(const double *phi, int n_patch, int64_t n_nodes,
+                           int M, const int32_t *grid, double h, double x0,
+                           double y0, float *feat, double *proj,
+                           int64_t *node, int64_t max_samples,
+                           int64_t *n_samples, void *stream);
+
+typedef struct casl_mlp casl_mlp_t;
+/* Fully connected ReLU network with widths dims[0..n_layers] (no ReLU on the
+ * output), He-normal initialisation from seed, zero biases. */
+int casl_mlp_create(const int32_t *dims, int n_layers, uint64_t seed,
+                    casl_mlp_t **out);
+int casl_mlp_destroy(casl_mlp_t *m);
+int casl_mlp_num_params(const casl_mlp_t *m, int64_t *n_weights,
+                        int64_t *n_biases);
+/* input standardisation x -> (x - mean)/stdev (host arrays of dims[0]) */
+int casl_mlp_set_scaler(casl_mlp_t *m, const float *mean, const float *stdev);
+/* host arrays: weights per layer [in][out] concatenated, biases per layer */
+int casl_mlp_get_params(const casl_mlp_t *m, float *weights, float *biases);
+int casl_mlp_set_params(casl_mlp_t *m, const float *weights,
+                        const float *biases);
+int casl_mlp_get_grads(const casl_mlp_t *m, float *grad_weights,
+                       float *grad_biases);
+/* y[n][dims[L]] = net(x[n][dims[0]]) */
+int casl_mlp_forward(casl_mlp_t *m, const float *x, int n, float *y,
+                     void *stream);
+/* gradients of mean((y - t)^2) w.r.t. all parameters; *loss is host */
+int casl_mlp_backward(casl_mlp_t *m, const float *x, const float *t, int n,
+                      float *loss, void *stream);
+int casl_mlp_adam_step(casl_mlp_t *m, float lr, float beta1, float beta2,
+                       float eps, void *stream);
+/* C = op(A) op(B) (+ bias[N]) (ReLU), BF16 operands (raw uint16), FP32
+ * accumulation; C32 and/or C16 (BF16) written with leading dimension ldc.
+ * op(A)(i,k) = trans_a ? A[k*lda+i] : A[i*lda+k]; op(B) likewise. */
+int casl_bf16_gemm(int M, int N, int K, const uint16_t *A, int lda,
+                   int trans_a, const uint16_t *B, int ldb, int trans_b,
+                   const float *bias, int relu, float *C32, uint16_t *C16,
+                   int ldc, void *stream);
+
 /* ------------------------------------------------ host forest (p4est role)
  * A single tree on the box [box_min, box_max] (one brick tree, non periodic). */
 typedef struct casl_forest casl_forest_t;

@@ -114,6 +114,23 @@ def kernel_profile(fn, torch):
     return out
 
 
+def pmc_traffic(kernel, workload, nodes):
+    """HBM bytes per launch of the dominant kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (calibrated against a copy kernel of known
+    size in the same run), when it was taken on this workload; else None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload}_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith('{"traffic"'):
+            t = json.loads(line)["traffic"]
+            if t.get("workload") == workload and t.get("nodes") == nodes:
+                return dict(bytes_per_launch=t["bytes_per_launch"], source=os.path.relpath(path, ROOT),
+                            fetch_kib=t["fetch_kib"], write_kib=t["write_kib"],
+                            fetch_factor=t["fetch_factor"], write_factor=t["write_factor"])
+    return None
+
+
 def cpu_baseline(W, budget_s):
     from oracle import pyoracle as O
     tab, phi0, iters = W["ngbd"].tables, W["phi0"], W["iters"]
@@ -197,9 +214,14 @@ def main():
     km = prof["mixed"]
     dom = max((k for k in km if km[k]["alg_bytes_per_launch"]), key=lambda k: km[k]["total_ms"])
     achieved = km[dom]["achieved_GBps"] * 1e9
+    tr = pmc_traffic(dom, args.config, N) if args.path == "tiled" else None
     roofline = dict(bound="hbm", kernel=dom, achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=None,
+                    frac=achieved / HBM_PEAK, traffic=tr["bytes_per_launch"] if tr else None,
                     alg_bytes_per_launch=km[dom]["alg_bytes_per_launch"], avg_launch_ms=km[dom]["avg_ms"])
+    if tr:
+        roofline["traffic_source"] = tr["source"]
+        roofline["traffic_over_alg"] = tr["bytes_per_launch"] / km[dom]["alg_bytes_per_launch"]
+        roofline["hbm_GBps_at_traffic"] = tr["bytes_per_launch"] / (km[dom]["avg_ms"] * 1e-3) / 1e9
     eff = dict(mixed=rate_mixed * b_alg(dim, 4) / HBM_PEAK, fp64=rate_fp64 * b_alg(dim, 8) / HBM_PEAK,
                b_alg_mixed=b_alg(dim, 4), b_alg_fp64=b_alg(dim, 8))
 

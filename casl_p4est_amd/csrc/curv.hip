@@ -106,6 +106,75 @@ __global__ void k_samples(Patch P, int n_patch, const double *__restrict__ nrm, 
 
 __global__ void k_zero_count(unsigned long long *c) { *c = 0ull; }
 
+// Network inputs of a sample (kml::Curvature::_computeHybridHK,
+// src/my_p4est_curvature_ml.cpp:483-520): negate every feature when the
+// numerical h*kappa is positive, rotate the stencil by multiples of 90 degrees
+// so that the gradient at the centre points into the first quadrant
+// (rotateStencilToFirstQuadrant, :373-398; rotateStencil90, :327-353; a zero
+// gradient component counts as +EPS), and emit it twice: as is (row 2s) and
+// reflected about y = x (row 2s+1, reflectStencil_yEqx :356-370).  The
+// gradient is the central difference of the phi stencil (the reference
+// carries normal stencils for it; only their direction matters here).
+__global__ void k_normalize(const float *__restrict__ feat, int64_t n, float *__restrict__ out) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  float f[10], g[9];
+#pragma unroll
+  for (int q = 0; q < 10; ++q) f[q] = feat[10 * s + q];
+  const float sg = f[9] > 0.f ? -1.f : 1.f;
+#pragma unroll
+  for (int q = 0; q < 10; ++q) f[q] *= sg;
+  float gx = f[7] - f[1], gy = f[5] - f[3];
+  if (gx == 0.f) gx = float(dev::EPS);
+  if (gy == 0.f) gy = float(dev::EPS);
+  // index maps of the rotated stencil: new[q] = old[map[q]]
+  constexpr int CCW[9] = {2, 5, 8, 1, 4, 7, 0, 3, 6};  // +90 degrees
+  constexpr int CW[9] = {6, 3, 0, 7, 4, 1, 8, 5, 2};   // -90 degrees
+  constexpr int HALF[9] = {8, 7, 6, 5, 4, 3, 2, 1, 0};  // 180 degrees
+  const int quad = gy > 0.f ? (gx > 0.f ? 1 : 2) : (gx < 0.f ? 3 : 4);
+#pragma unroll
+  for (int q = 0; q < 9; ++q) g[q] = f[q];
+  if (quad == 2) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) f[q] = g[CW[q]];
+  } else if (quad == 3) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) f[q] = g[HALF[q]];
+  } else if (quad == 4) {
+#pragma unroll
+    for (int q = 0; q < 9; ++q) f[q] = g[CCW[q]];
+  }
+  float *o = out + 20 * s;
+#pragma unroll
+  for (int q = 0; q < 10; ++q) o[q] = f[q];
+  constexpr int REFL[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};  // swap 1-3, 2-6, 5-7
+#pragma unroll
+  for (int q = 0; q < 9; ++q) o[10 + q] = f[REFL[q]];
+  o[19] = f[9];
+}
+
+// Hybrid h*kappa (src/my_p4est_curvature_ml.cpp:483-556): the numerical value
+// where |ihk| < lo; elsewhere the mean of the two network outputs (plus the
+// normalised numerical value -|ihk| when the network predicts a correction),
+// blended linearly towards -|ihk| for |ihk| <= hi, and given the sign of ihk.
+__global__ void k_hybrid(const float *__restrict__ feat, const float *__restrict__ pred, int64_t n, float lo,
+                         float hi, int residual, float *__restrict__ hk) {
+  const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const float ihk = feat[10 * s + 9], a = fabsf(ihk);
+  if (a < lo) {
+    hk[s] = ihk;
+    return;
+  }
+  float p = 0.5f * (pred[2 * s] + pred[2 * s + 1]);
+  if (residual) p += -a;
+  if (a <= hi) {
+    const float lam = (hi - a) / (hi - lo);
+    p = (1.f - lam) * p + lam * -a;
+  }
+  hk[s] = ihk > 0.f ? fabsf(p) : -fabsf(p);
+}
+
 }  // namespace
 
 extern "C" {
@@ -136,6 +205,28 @@ int casl_curvature_samples(const double *phi, int n_patch, int64_t n_nodes, int 
     HIP_CHECK(hipMemcpyAsync(&c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, s));
     HIP_CHECK(hipStreamSynchronize(s));
     *n_samples = int64_t(c);
+  });
+}
+
+int casl_curvature_normalize(const float *feat, int64_t n, float *out, void *stream) {
+  return guarded([&]() {
+    if (n < 0 || (n > 0 && (!feat || !out))) throw std::invalid_argument("bad arguments");
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_normalize, dim3(unsigned((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       feat, n, out);
+    HIP_CHECK(hipGetLastError());
+  });
+}
+
+int casl_curvature_hybrid(const float *feat, const float *pred, int64_t n, float lo, float hi, int residual,
+                          float *hk, void *stream) {
+  return guarded([&]() {
+    if (n < 0 || (n > 0 && (!feat || !pred || !hk)) || !(hi > lo) || !(lo >= 0.f))
+      throw std::invalid_argument("bad arguments");
+    if (n == 0) return;
+    hipLaunchKernelGGL(k_hybrid, dim3(unsigned((n + 255) / 256)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                       feat, pred, n, lo, hi, residual, hk);
+    HIP_CHECK(hipGetLastError());
   });
 }
 

@@ -50,32 +50,58 @@ __host__ __device__ inline float bf2f(uint16_t h) {
 //   op(A)(i, k) = TA ? A[k*lda + i] : A[i*lda + k]
 //   op(B)(k, j) = TB ? B[j*ldb + k] : B[k*ldb + j]
 //   mask: multiply by (mask[i*ldm + j] > 0), the ReLU derivative of a stored activation
+// Split K: block z covers k in [z*kc, (z+1)*kc) and writes its FP32 partial
+// to C32 + z*M*ldc (k_sum_splits adds the partials in a fixed order).
 template <bool TA, bool TB>
 __global__ void __launch_bounds__(256)
-    k_gemm(int M, int N, int K, const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B, int ldb,
-           const float *__restrict__ bias, const uint16_t *__restrict__ mask, int ldm, int relu,
+    k_gemm(int M, int N, int K, int kc, const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B,
+           int ldb, const float *__restrict__ bias, const uint16_t *__restrict__ mask, int ldm, int relu,
            float *__restrict__ C32, uint16_t *__restrict__ C16, int ldc) {
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int tm = blockIdx.y * 64 + (wave >> 1) * 32;
   const int tn = blockIdx.x * 64 + (wave & 1) * 32;
   const int li = lane & 31, lk = (lane >> 5) * 8;
   const int i = tm + li, j = tn + li;
+  const int kb = int(blockIdx.z) * kc, ke = min(K, kb + kc);
+  // rows of op(A) / columns of op(B) that are contiguous in k can be read 16 bytes at a time
+  const bool avec = !TA && (lda & 7) == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+  const bool bvec = TB && (ldb & 7) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
   f32x16 acc;
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  for (int k0 = 0; k0 < K; k0 += 16) {
+  for (int k0 = kb; k0 < ke; k0 += 16) {
+    const int kk = k0 + lk;
+    union {
+      uint4 q;
+      uint16_t h[8];
+    } ua, ub;
+    if (avec && i < M && kk + 8 <= ke) {
+      ua.q = *reinterpret_cast<const uint4 *>(A + size_t(i) * lda + kk);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int k = kk + v;
+        ua.h[v] = (i < M && k < ke) ? (TA ? A[size_t(k) * lda + i] : A[size_t(i) * lda + k]) : uint16_t(0);
+      }
+    }
+    if (bvec && j < N && kk + 8 <= ke) {
+      ub.q = *reinterpret_cast<const uint4 *>(B + size_t(j) * ldb + kk);
+    } else {
+#pragma unroll
+      for (int v = 0; v < 8; ++v) {
+        const int k = kk + v;
+        ub.h[v] = (j < N && k < ke) ? (TB ? B[size_t(j) * ldb + k] : B[size_t(k) * ldb + j]) : uint16_t(0);
+      }
+    }
     bf16x8 a, b;
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      const int k = k0 + lk + v;
-      uint16_t av = 0, bv = 0;
-      if (i < M && k < K) av = TA ? A[size_t(k) * lda + i] : A[size_t(i) * lda + k];
-      if (j < N && k < K) bv = TB ? B[size_t(j) * ldb + k] : B[size_t(k) * ldb + j];
-      a[v] = __builtin_bit_cast(__bf16, av);
-      b[v] = __builtin_bit_cast(__bf16, bv);
+      a[v] = __builtin_bit_cast(__bf16, ua.h[v]);
+      b[v] = __builtin_bit_cast(__bf16, ub.h[v]);
     }
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
+  float *C = C32 ? C32 + size_t(blockIdx.z) * M * ldc : nullptr;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int row = tm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -85,10 +111,19 @@ __global__ void __launch_bounds__(256)
       if (bias) v += bias[col];
       if (relu) v = v > 0.f ? v : 0.f;
       if (mask && !(bf2f(mask[size_t(row) * ldm + col]) > 0.f)) v = 0.f;
-      if (C32) C32[size_t(row) * ldc + col] = v;
+      if (C) C[size_t(row) * ldc + col] = v;
       if (C16) C16[size_t(row) * ldc + col] = f2bf(v);
     }
   }
+}
+
+// out[e] = sum_z part[z*n + e], z = 0 .. splits-1 in order
+__global__ void k_sum_splits(const float *__restrict__ part, int splits, int64_t n, float *__restrict__ out) {
+  const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  float s = part[e];
+  for (int z = 1; z < splits; ++z) s += part[size_t(z) * n + e];
+  out[e] = s;
 }
 
 // A_0 = bf16((x - mean) / std)
@@ -160,13 +195,23 @@ inline unsigned blocks(int64_t n, int b = 256) { return unsigned((n + b - 1) / b
 
 template <bool TA, bool TB>
 void gemm(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, const float *bias,
-          const uint16_t *mask, int ldm, int relu, float *C32, uint16_t *C16, int ldc, hipStream_t s) {
-  dim3 grid(unsigned((N + 63) / 64), unsigned((M + 63) / 64));
+          const uint16_t *mask, int ldm, int relu, float *C32, uint16_t *C16, int ldc, hipStream_t s,
+          int splits = 1) {
+  const int kc = splits > 1 ? ((K + splits - 1) / splits + 15) / 16 * 16 : K;
+  dim3 grid(unsigned((N + 63) / 64), unsigned((M + 63) / 64), unsigned(splits));
   prof_begin("k_gemm", s);
-  hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, s, M, N, K, A, lda, B, ldb, bias, mask, ldm, relu, C32,
-                     C16, ldc);
+  hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, s, M, N, K, kc, A, lda, B, ldb, bias, mask, ldm, relu,
+                     C32, C16, ldc);
   prof_end("k_gemm", s);
   HIP_CHECK(hipGetLastError());
+}
+
+// number of K splits that gives a small output tile grid enough blocks
+inline int k_splits(int M, int N, int K) {
+  const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+  int s = 1;
+  while (s < 64 && tiles * s < 512 && K / (2 * s) >= 256) s *= 2;
+  return s;
 }
 
 }  // namespace
@@ -183,6 +228,7 @@ struct casl_mlp {
   DevBuf<float> z32;                // FP32 scratch [cap x max width]
   DevBuf<uint16_t> d16a, d16b;      // BF16 back-propagated gradients
   DevBuf<float> y32, loss;
+  DevBuf<float> part;               // split-K partial sums of weight gradients
   std::vector<size_t> offA;
   int maxw = 0;
   int64_t step = 0;
@@ -359,7 +405,16 @@ int casl_mlp_backward(casl_mlp_t *m, const float *x, const float *t, int n, floa
       HIP_CHECK(hipGetLastError());
       const uint16_t *Aprev = m->act16.p + m->offA[l];  // [n x in]
       // dW_l = A_{l-1}^T dZ_l : (in x n)(n x o)
-      gemm<true, false>(in, o, n, Aprev, in, dcur, o, nullptr, nullptr, 0, 0, m->gW.p + m->offW[l], nullptr, o, s);
+      const int sp = k_splits(in, o, n);
+      if (sp == 1) {
+        gemm<true, false>(in, o, n, Aprev, in, dcur, o, nullptr, nullptr, 0, 0, m->gW.p + m->offW[l], nullptr, o, s);
+      } else {
+        m->part.ensure(size_t(sp) * in * o);
+        gemm<true, false>(in, o, n, Aprev, in, dcur, o, nullptr, nullptr, 0, 0, m->part.p, nullptr, o, s, sp);
+        const int64_t e = int64_t(in) * o;
+        hipLaunchKernelGGL(k_sum_splits, dim3(blocks(e)), dim3(256), 0, s, m->part.p, sp, e, m->gW.p + m->offW[l]);
+        HIP_CHECK(hipGetLastError());
+      }
       if (l > 0) {
         // dZ_{l-1} = (dZ_l W_l^T) * relu'(A_{l-1}) : (n x o)(o x in)
         gemm<false, true>(n, in, o, dcur, o, m->W16.p + m->offW[l], o, nullptr, Aprev, in, 0, m->z32.p, dnext, in, s);

@@ -190,6 +190,21 @@ int casl_curvature_samples(const double *phi, int n_patch, int64_t n_nodes,
                            int64_t *node, int64_t max_samples,
                            int64_t *n_samples, void *stream);
 
+/* Network inputs of n samples (feat [n][10] as above): out[2s] = the sample
+ * negated when feat[s][9] > 0 and rotated so its centre gradient lies in the
+ * first quadrant, out[2s+1] = that reflected about y = x.  Replaces the input
+ * preparation of kml::Curvature::_computeHybridHK
+ * (src/my_p4est_curvature_ml.cpp:483-520, :327-398). */
+int casl_curvature_normalize(const float *feat, int64_t n, float *out,
+                             void *stream);
+/* Hybrid h*kappa from the network outputs pred[2n] of the normalised inputs
+ * (src/my_p4est_curvature_ml.cpp:520-556): feat[s][9] where |feat[s][9]| < lo,
+ * otherwise the mean prediction (+ -|ihk| when residual != 0) blended towards
+ * -|ihk| up to hi, with the sign of feat[s][9]. */
+int casl_curvature_hybrid(const float *feat, const float *pred, int64_t n,
+                          float lo, float hi, int residual, float *hk,
+                          void *stream);
+
 typedef struct casl_mlp casl_mlp_t;
 /* Fully connected ReLU network with widths dims[0..n_layers] (no ReLU on the
  * output), He-normal initialisation from seed, zero biases. */

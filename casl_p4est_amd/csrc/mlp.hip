@@ -391,6 +391,7 @@ int casl_mlp_backward(casl_mlp_t *m, const float *x, const float *t, int n, floa
     if (!m || !x || !t || n <= 0) throw std::invalid_argument("bad arguments");
     if (m->dim[m->L] != 1) throw std::invalid_argument("training needs a scalar output");
     hipStream_t s = static_cast<hipStream_t>(stream);
+    m->ensure_cap(n);  // sizes y32 before its address is taken
     forward(m, x, n, m->y32.p, s);
     hipLaunchKernelGGL(k_zero_f, dim3(1), dim3(64), 0, s, m->loss.p, 1);
     // dZ_L = 2 (y - t) / n  -> z32 (FP32) and d16a (BF16), [n x 1]

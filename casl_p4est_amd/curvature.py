@@ -1,0 +1,359 @@
+"""Learned h*kappa: numerical curvature samples, the BF16-MFMA network, its
+training on GPU-reinitialized level sets, and the hybrid estimate.
+
+The reference's counterpart is kml::Curvature (src/my_p4est_curvature_ml.cpp,
+src/my_p4est_curvature_ml.h): it samples nodes next to the interface, feeds
+their (sign-normalised, rotated, reflected) phi stencils and the numerical
+h*kappa to a network, and blends with the numerical value where |h*kappa| is
+small (_computeHybridHK, :483-556).  Here every step runs on the GPU through
+the library's own kernels:
+
+    UniformPatch            a (2^L)^2 uniform grid on [0, size]^2 as one tree
+    make_dataset            circles and sinusoids -> phi0 -> reinitialize_2nd_order
+                            (FP64, 20 iterations) -> casl_curvature_samples,
+                            with exact h*kappa targets at the closest interface point
+    CurvatureNet            casl_mlp_* (BF16 MFMA GEMMs, FP32 accumulation, Adam)
+    train                   mini-batch Adam on normalised inputs
+    hybrid_hk               casl_curvature_normalize -> network -> casl_curvature_hybrid
+
+The network predicts a correction to the numerical value (residual=True):
+h*kappa = -|ihk| + net(x) for a normalised input x; the hybrid estimate uses
+the numerical h*kappa alone where |ihk| < 0.004 and blends up to 0.007, as the
+reference does.
+"""
+import ctypes
+import math
+import time
+
+import numpy as np
+
+from . import _lib as L
+from .forest import Forest, NodeNeighbors
+from .level_set import LevelSet, _stream_ptr, _torch
+from .levelset import Plane
+
+LO_MIN_HK = 0.004
+UP_MIN_HK = 0.007
+DIMS_2D = (10, 128, 128, 128, 128, 1)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class UniformPatch:
+    """Uniform grid of 2^level cells per side on [0, size]^2 (one tree, built
+    by refining to min = max level), with its device stencil and the map from
+    grid position (i, j) to node index."""
+
+    def __init__(self, level=8, size=2.0):
+        torch = _torch()
+        self.level, self.size = level, float(size)
+        self.M = 2 ** level
+        self.h = self.size / self.M
+        f = Forest(2, [0.0, 0.0], [self.size, self.size]).refine_levelset(Plane([1.0, 0.0], -0.5 * self.size),
+                                                                         level, level)
+        f.build_nodes()
+        self.forest = f
+        self.ngbd = NodeNeighbors(f).init_neighbors()
+        self.n_nodes = self.ngbd.num_nodes
+        if self.n_nodes != (self.M + 1) ** 2:
+            raise RuntimeError("uniform patch has an unexpected node count")
+        _, xyz = f.nodes()
+        ij = np.rint(xyz / self.h).astype(np.int64)
+        grid = np.full((self.M + 1) * (self.M + 1), -1, dtype=np.int32)
+        grid[ij[:, 0] * (self.M + 1) + ij[:, 1]] = np.arange(self.n_nodes, dtype=np.int32)
+        if (grid < 0).any():
+            raise RuntimeError("uniform patch nodes do not cover the grid")
+        self.xy = xyz
+        self.grid = torch.from_numpy(grid).cuda()
+        self.solver = LevelSet(self.ngbd)
+
+    def reinitialize(self, phi, iterations=20, precision="fp64"):
+        """reinitialize_2nd_order on every row of phi [n_patch][n_nodes]."""
+        for p in range(phi.shape[0]):
+            self.solver.reinitialize_2nd_order(phi[p], iterations, precision=precision)
+        return phi
+
+    def samples(self, phi, max_samples=None):
+        """casl_curvature_samples on phi [n_patch][n_nodes] (float64, device).
+        Returns (feat [n][10] float32, proj [n][2] float64, node [n] int64) on
+        the device, sorted by node (patch-major)."""
+        torch = _torch()
+        n_patch = phi.shape[0]
+        cap = max_samples or max(1024, n_patch * 16 * self.M)
+        while True:
+            feat = torch.empty((cap, 10), dtype=torch.float32, device=phi.device)
+            proj = torch.empty((cap, 2), dtype=torch.float64, device=phi.device)
+            node = torch.empty(cap, dtype=torch.int64, device=phi.device)
+            cnt = ctypes.c_int64()
+            L.call("casl_curvature_samples", _ptr(phi), n_patch, self.n_nodes, self.M, _ptr(self.grid), self.h, 0.0,
+                   0.0, _ptr(feat), _ptr(proj), _ptr(node), cap, ctypes.byref(cnt), _stream_ptr(torch))
+            if cnt.value <= cap:
+                break
+            cap = cnt.value
+        n = cnt.value
+        order = torch.argsort(node[:n])
+        return feat[:n][order], proj[:n][order], node[:n][order]
+
+
+# ------------------------------------------------------------------ shapes
+def circle_phi0(xy, c, R):
+    """(|x - c|^2 - R^2) / (2 R): negative inside, not a signed distance."""
+    d2 = (xy[:, 0] - c[0]) ** 2 + (xy[:, 1] - c[1]) ** 2
+    return (d2 - R * R) / (2.0 * R)
+
+
+def sinusoid_frame(xy, c, theta):
+    ct, st = math.cos(theta), math.sin(theta)
+    dx, dy = xy[..., 0] - c[0], xy[..., 1] - c[1]
+    return ct * dx + st * dy, -st * dx + ct * dy
+
+
+def sinusoid_phi0(xy, c, theta, A, omega, phase):
+    """v - A sin(omega u + phase) in the frame rotated by theta about c."""
+    u, v = sinusoid_frame(xy, c, theta)
+    return v - A * np.sin(omega * u + phase)
+
+
+def sinusoid_closest_hk(points, c, theta, A, omega, phase, h):
+    """Exact h*kappa at the point of the curve closest to each point (brute
+    force over +-3h along the curve parameter, then Newton)."""
+    u, v = sinusoid_frame(points, c, theta)
+    s = u[:, None] + np.linspace(-3 * h, 3 * h, 97)[None, :]
+    d2 = (s - u[:, None]) ** 2 + (A * np.sin(omega * s + phase) - v[:, None]) ** 2
+    s = s[np.arange(len(u)), np.argmin(d2, axis=1)]
+    for _ in range(8):
+        ph = omega * s + phase
+        f, f1, f2 = A * np.sin(ph), A * omega * np.cos(ph), -A * omega * omega * np.sin(ph)
+        g1 = (s - u) + (f - v) * f1
+        g2 = 1.0 + f1 * f1 + (f - v) * f2
+        s = s - g1 / np.where(g2 > 0.1, g2, 0.1)
+    ph = omega * s + phase
+    f1, f2 = A * omega * np.cos(ph), -A * omega * omega * np.sin(ph)
+    return h * (-f2 / (1.0 + f1 * f1) ** 1.5)
+
+
+def make_dataset(patch, n_circles, n_sinusoids, seed=0, iterations=20, chunk=32, log=None):
+    """Samples near random circles (radius U[1.6h, 0.5], centre jittered by
+    U[-h/2, h/2]^2 about the patch centre) and sinusoids (A U[0.01, 0.2],
+    omega U[2 pi, 12 pi], random phase, rotation and offset), each phi0
+    reinitialized by the GPU solver.  Returns dict(feat (device float32
+    [n][10]), target (numpy float64 exact h*kappa), kind (0 circle, 1 sinusoid),
+    timings)."""
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    h, mid = patch.h, 0.5 * patch.size
+    shapes = []
+    for _ in range(n_circles):
+        shapes.append(("circle", (mid + rng.uniform(-h / 2, h / 2), mid + rng.uniform(-h / 2, h / 2)),
+                       rng.uniform(1.6 * h, 0.5)))
+    for _ in range(n_sinusoids):
+        omega = rng.uniform(2 * math.pi, 12 * math.pi)
+        shapes.append(("sinusoid", (mid + rng.uniform(-h / 2, h / 2), mid + rng.uniform(-h / 2, h / 2)),
+                       rng.uniform(0, 2 * math.pi), rng.uniform(0.01, 0.2), omega, rng.uniform(0, 2 * math.pi)))
+    feats, targets, kinds = [], [], []
+    t_reinit = t_samples = 0.0
+    for c0 in range(0, len(shapes), chunk):
+        group = shapes[c0:c0 + chunk]
+        phi_np = np.empty((len(group), patch.n_nodes))
+        for k, s in enumerate(group):
+            phi_np[k] = circle_phi0(patch.xy, s[1], s[2]) if s[0] == "circle" else sinusoid_phi0(patch.xy, *s[1:])
+        phi = torch.from_numpy(phi_np).cuda()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        patch.reinitialize(phi, iterations)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        feat, proj, node = patch.samples(phi)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_reinit += t1 - t0
+        t_samples += t2 - t1
+        which = (node // patch.n_nodes).cpu().numpy()
+        proj_np = proj.cpu().numpy()
+        tgt = np.empty(len(which))
+        for k, s in enumerate(group):
+            sel = which == k
+            if s[0] == "circle":
+                tgt[sel] = h / s[2]
+            else:
+                tgt[sel] = sinusoid_closest_hk(proj_np[sel], *s[1:], h)
+        feats.append(feat)
+        targets.append(tgt)
+        kinds.append(np.array([0 if group[k][0] == "circle" else 1 for k in which], dtype=np.int8))
+        if log:
+            log(f"[curvature] shapes {c0 + len(group)}/{len(shapes)}: {sum(len(t) for t in targets)} samples")
+    return dict(feat=torch.cat(feats), target=np.concatenate(targets), kind=np.concatenate(kinds),
+                reinit_s=t_reinit, samples_s=t_samples)
+
+
+# ------------------------------------------------------------------ network
+class CurvatureNet:
+    """casl_mlp: fully connected ReLU network, BF16 MFMA GEMMs with FP32
+    accumulation, FP32 master weights, Adam."""
+
+    def __init__(self, dims=DIMS_2D, seed=0, zero_last_layer=True):
+        _torch().cuda.init()
+        self.dims = tuple(int(d) for d in dims)
+        d = np.asarray(self.dims, dtype=np.int32)
+        h = ctypes.c_void_p()
+        L.call("casl_mlp_create", d.ctypes.data_as(L.c_int32_p), len(self.dims) - 1, int(seed), ctypes.byref(h))
+        self._h = h
+        nw, nb = ctypes.c_int64(), ctypes.c_int64()
+        L.call("casl_mlp_num_params", h, ctypes.byref(nw), ctypes.byref(nb))
+        self.n_weights, self.n_biases = nw.value, nb.value
+        self.mean = np.zeros(self.dims[0], np.float32)
+        self.std = np.ones(self.dims[0], np.float32)
+        if zero_last_layer:
+            # a correction network then starts at zero: the numerical value
+            w, b = self.params()
+            w[-self.dims[-2] * self.dims[-1]:] = 0.0
+            self.set_params(w, b)
+
+    def scale_output(self, factor):
+        """Multiply the network output by factor (folded into the last layer)."""
+        w, b = self.params()
+        w[-self.dims[-2] * self.dims[-1]:] *= np.float32(factor)
+        b[-self.dims[-1]:] *= np.float32(factor)
+        self.set_params(w, b)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            L.lib().casl_mlp_destroy(h)
+            self._h = None
+
+    def set_scaler(self, mean, std):
+        self.mean = np.ascontiguousarray(mean, dtype=np.float32)
+        self.std = np.ascontiguousarray(std, dtype=np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.call("casl_mlp_set_scaler", self._h, self.mean.ctypes.data_as(fp), self.std.ctypes.data_as(fp))
+
+    def params(self):
+        w = np.empty(self.n_weights, np.float32)
+        b = np.empty(self.n_biases, np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.call("casl_mlp_get_params", self._h, w.ctypes.data_as(fp), b.ctypes.data_as(fp))
+        return w, b
+
+    def set_params(self, w, b):
+        w = np.ascontiguousarray(w, np.float32)
+        b = np.ascontiguousarray(b, np.float32)
+        if w.size != self.n_weights or b.size != self.n_biases:
+            raise ValueError("parameter sizes do not match the network")
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.call("casl_mlp_set_params", self._h, w.ctypes.data_as(fp), b.ctypes.data_as(fp))
+
+    def grads(self):
+        w = np.empty(self.n_weights, np.float32)
+        b = np.empty(self.n_biases, np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.call("casl_mlp_get_grads", self._h, w.ctypes.data_as(fp), b.ctypes.data_as(fp))
+        return w, b
+
+    def _check(self, x):
+        torch = _torch()
+        if not isinstance(x, torch.Tensor) or x.device.type != "cuda" or x.dtype != torch.float32:
+            raise TypeError("inputs must be a float32 CUDA tensor")
+        if x.dim() != 2 or x.shape[1] != self.dims[0] or not x.is_contiguous():
+            raise ValueError(f"inputs must be contiguous [n][{self.dims[0]}]")
+
+    def forward(self, x, out=None):
+        torch = _torch()
+        self._check(x)
+        if out is None:
+            out = torch.empty((x.shape[0], self.dims[-1]), dtype=torch.float32, device=x.device)
+        L.call("casl_mlp_forward", self._h, _ptr(x), int(x.shape[0]), _ptr(out), _stream_ptr(torch))
+        return out
+
+    def backward(self, x, t):
+        """Forward + backward of mean((y - t)^2); returns the loss."""
+        torch = _torch()
+        self._check(x)
+        if t.dtype != torch.float32 or t.numel() != x.shape[0] or not t.is_contiguous():
+            raise ValueError("targets must be a contiguous float32 tensor with one value per row")
+        loss = ctypes.c_float()
+        L.call("casl_mlp_backward", self._h, _ptr(x), _ptr(t), int(x.shape[0]), ctypes.byref(loss),
+               _stream_ptr(torch))
+        return loss.value
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        L.call("casl_mlp_adam_step", self._h, float(lr), float(beta1), float(beta2), float(eps),
+               _stream_ptr(_torch()))
+
+
+def normalize(feat):
+    """casl_curvature_normalize: [n][10] samples -> [2n][10] network inputs."""
+    torch = _torch()
+    out = torch.empty((2 * feat.shape[0], 10), dtype=torch.float32, device=feat.device)
+    L.call("casl_curvature_normalize", _ptr(feat), int(feat.shape[0]), _ptr(out), _stream_ptr(torch))
+    return out
+
+
+def hybrid(feat, pred, lo=LO_MIN_HK, hi=UP_MIN_HK, residual=True):
+    torch = _torch()
+    hk = torch.empty(feat.shape[0], dtype=torch.float32, device=feat.device)
+    L.call("casl_curvature_hybrid", _ptr(feat), _ptr(pred), int(feat.shape[0]), float(lo), float(hi),
+           int(bool(residual)), _ptr(hk), _stream_ptr(torch))
+    return hk
+
+
+def training_pairs(feat, target):
+    """Normalised inputs [2n][10] and their targets: the exact h*kappa with
+    the sign convention of the normalised stencil, minus the normalised
+    numerical value -|ihk| (the network learns the correction)."""
+    torch = _torch()
+    x = normalize(feat)
+    t = torch.as_tensor(target, dtype=torch.float64, device=feat.device)
+    ihk = feat[:, 9].double()
+    tn = torch.where(ihk > 0, -t, t) + ihk.abs()
+    return x, tn.float().repeat_interleave(2).contiguous()
+
+
+def train(net, feat, target, epochs=10, batch=16384, lr=1e-3, lr_final=1e-5, seed=0, log=None):
+    """Adam on normalised samples with a cosine learning-rate decay.  The
+    input scaler comes from the data; the targets are trained at unit variance
+    and that scale is folded into the last layer at the end.  Returns
+    per-epoch mean losses (in the unit-variance scale) and timing."""
+    torch = _torch()
+    x, t = training_pairs(feat, target)
+    mean = x.mean(dim=0).cpu().numpy()
+    std = x.std(dim=0).clamp_min(1e-6).cpu().numpy()
+    net.set_scaler(mean, std)
+    sigma = float(t.std().clamp_min(1e-12))
+    t = (t / sigma).contiguous()
+    n = x.shape[0]
+    steps_per_epoch = max(1, n // batch)
+    total = epochs * steps_per_epoch
+    g = torch.Generator(device=x.device)
+    g.manual_seed(seed)
+    hist = []
+    step = 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        perm = torch.randperm(n, device=x.device, generator=g)
+        acc = 0.0
+        for k in range(steps_per_epoch):
+            idx = perm[k * batch:(k + 1) * batch]
+            xb = x.index_select(0, idx)
+            tb = t.index_select(0, idx)
+            acc += net.backward(xb, tb)
+            cur = lr_final + 0.5 * (lr - lr_final) * (1 + math.cos(math.pi * step / max(1, total - 1)))
+            net.adam_step(cur)
+            step += 1
+        hist.append(acc / steps_per_epoch)
+        if log:
+            log(f"[curvature] epoch {ep + 1}/{epochs} loss {hist[-1]:.3e}")
+    torch.cuda.synchronize()
+    seconds = time.perf_counter() - t0
+    net.scale_output(sigma)
+    return dict(loss=hist, seconds=seconds, steps=step, rows=step * batch, target_scale=sigma)
+
+
+def hybrid_hk(net, feat, residual=True):
+    """Learned h*kappa of samples feat [n][10] (device)."""
+    x = normalize(feat)
+    y = net.forward(x)
+    return hybrid(feat, y, residual=residual)

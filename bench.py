@@ -85,6 +85,141 @@ def build_workload(name, lmax_override=None):
     return dict(dim=dim, lmin=lmin, lmax=lmax, iters=iters, forest=forest, ngbd=ngbd, phi0=phi0, desc=desc)
 
 
+def build_batch_workload(n_trees=4096, threads=16):
+    """4096 independent quadtrees on the unit square: lmin 3, lmax U{6..9},
+    random star interfaces (centre U[0.35,0.65]^2, r0 U[0.15,0.3], amplitude
+    U[0,0.08], lobes U{3..8}) with phi0 scaled by (1 + 0.8 |x-c|^2)."""
+    from concurrent.futures import ThreadPoolExecutor
+    import casl_p4est_amd as C
+    from casl_p4est_amd import levelset as ls
+    rng = np.random.default_rng(SEED)
+    specs = [(ls.random_star(rng), int(rng.integers(6, 10))) for _ in range(n_trees)]
+
+    def one(spec):
+        shape, lmax = spec
+        f = C.Forest(2).refine_levelset(shape, 3, lmax, lip=2.4).build_nodes()
+        return C.flat_stencil_from_forest(f), f.sample(shape), float(np.min(f.dxyz_min()))
+
+    t0 = time.time()
+    with ThreadPoolExecutor(threads) as ex:
+        res = list(ex.map(one, specs))
+    log(f"[bench] host forests: {n_trees} trees in {time.time() - t0:.1f}s")
+    desc = (f"{n_trees} independent 2D quadtrees (unit square, lmin 3, lmax U{{6..9}}, random stars, "
+            f"phi0 x (1+0.8|x-c|^2), seed 20261202), reinitialize_2nd_order 20 iterations, one persistent launch")
+    return dict(tabs=[r[0] for r in res], phis=[r[1] for r in res], h=np.array([r[2] for r in res]),
+                lmax=np.array([s[1] for s in specs]), iters=20, desc=desc)
+
+
+def run_batch(args, torch):
+    """Batched trees: mixed precision, autotuned schedule, hipGraph replay;
+    against the per-tree FP64 loop of the plain single-tree path."""
+    import casl_p4est_amd as C
+    from concurrent.futures import ThreadPoolExecutor
+    dev = torch.device("cuda:0")
+    W = build_batch_workload()
+    iters = W["iters"]
+    batch = C.TreeBatch(W["tabs"])
+    N = batch.total_nodes
+    log(f"[bench] batch: {len(W['tabs'])} trees, {N} nodes, largest {int(batch.sizes.max())}")
+    phi0 = torch.from_numpy(np.concatenate(W["phis"])).to(dev)
+    phi = torch.empty_like(phi0)
+    bt, npt, ms = batch.autotune(phi0, iters, "mixed")
+    log(f"[bench] autotune: block {bt}, nodes/thread {npt}, {ms:.3f} ms")
+
+    def step_batch(precision, graph):
+        def run():
+            phi.copy_(phi0)
+            batch.reinitialize_2nd_order(phi, iters, precision=precision, graph=graph)
+        return run
+
+    t_mixed = time_steps(step_batch("mixed", True), args.steps, args.warmup, torch)
+    out_mixed = phi.cpu().numpy()
+    t_b64 = time_steps(step_batch("fp64", True), args.steps, args.warmup, torch)
+    out_b64 = phi.cpu().numpy()
+    rate_mixed = N * iters * args.steps / t_mixed
+    rate_b64 = N * iters * args.steps / t_b64
+    # mixed error against the FP64 result, per tree in its |phi| < 3 h band
+    err_h = 0.0
+    err = 0.0
+    for k in range(len(W["tabs"])):
+        a, b = batch.offsets[k], batch.offsets[k + 1]
+        ref = out_b64[a:b]
+        band = np.abs(ref) < 3 * W["h"][k]
+        if band.any():
+            e = float(np.max(np.abs(out_mixed[a:b][band] - ref[band])))
+            err = max(err, e)
+            err_h = max(err_h, e / W["h"][k])
+
+    # per-tree FP64 loop on the plain single-tree path
+    with ThreadPoolExecutor(16) as ex:
+        stencils = list(ex.map(C.DeviceStencil, W["tabs"]))
+    phis_t = [torch.from_numpy(p).to(dev) for p in W["phis"]]
+    work = [torch.empty_like(p) for p in phis_t]
+
+    def loop():
+        for st, p0, w in zip(stencils, phis_t, work):
+            w.copy_(p0)
+            st.reinit(w, 2, 2, iters, C.DBL_MAX, -C.DBL_MAX, 0)
+
+    t_loop = time_steps(loop, 1, 1, torch)
+    rate_loop = N * iters / t_loop
+    # the per-tree loop's FP64 results equal the batched FP64 results bit for bit
+    same = all(np.array_equal(w.cpu().numpy(), out_b64[batch.offsets[k]:batch.offsets[k + 1]])
+               for k, w in enumerate(work[:64]))
+    log(f"[bench] batch mixed {rate_mixed:.3e} node-iters/s, batch fp64 {rate_b64:.3e}, per-tree fp64 loop "
+        f"{rate_loop:.3e}; mixed err {err:.2e} (max {err_h:.2e} h); loop == batch fp64: {same}")
+
+    kp = kernel_profile(step_batch("mixed", False), torch)
+    ms_b, cnt = kp["k_batch"]
+    alg = b_alg(2, 4) * N * iters
+    achieved = alg / (ms_b / cnt * 1e-3)
+    roofline = dict(bound="hbm", kernel="k_batch", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
+                    frac=achieved / HBM_PEAK, traffic=None, alg_bytes_per_launch=alg, avg_launch_ms=ms_b / cnt,
+                    note="one launch = all 20 iterations of every tree; working set L2/MALL resident, so this "
+                         "is effective bandwidth")
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle as O
+        threads = O.num_threads()
+        t0 = time.perf_counter()
+        n_done = 0
+        k = 0
+        while time.perf_counter() - t0 < args.cpu_budget and k < len(W["tabs"]):
+            O.reinit(W["tabs"][k], W["phis"][k], 2, 2, iters)
+            n_done += W["tabs"][k].n_nodes
+            k += 1
+        dt = time.perf_counter() - t0
+        cpu = dict(value=n_done * iters / dt, unit="node-iters/s", cores=threads, kind="port",
+                   sample=f"first {k} of {len(W['tabs'])} trees, full 20 iterations each, FP64, one tree at a time "
+                          f"with {threads} OpenMP threads, {dt:.1f} s")
+        log(f"[bench] cpu baseline {cpu['value']:.3e} node-iters/s ({cpu['sample']})")
+    return {
+        "metric": METRIC,
+        "value": rate_mixed,
+        "unit": "node-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_mixed / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 bulk + f64 subcell/norm",
+        "data": "synthetic",
+        "config": {"workload": W["desc"], "trees": len(W["tabs"]), "nodes": N, "iterations": iters,
+                   "schedule": {"block": bt, "nodes_per_thread": npt}, "graph": True},
+        "fp64_batched": {"value": rate_b64, "unit": "node-iters/s"},
+        "fp64_per_tree_loop": {"value": rate_loop, "unit": "node-iters/s", "ms_per_step": 1e3 * t_loop,
+                               "bit_identical_to_batched_fp64": same},
+        "mixed_over_fp64_loop": rate_mixed / rate_loop,
+        "mixed_over_fp64_batched": rate_mixed / rate_b64,
+        "linf_err_band": err,
+        "linf_err_band_over_h": err_h,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+
+
 def time_steps(fn, steps, warmup, torch):
     for _ in range(warmup):
         fn()
@@ -155,7 +290,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d"])
+    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d", "batch2d"])
     ap.add_argument("--lmax", type=int, default=0, help="reduce max level (quick runs only)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -171,6 +306,9 @@ def main():
     import casl_p4est_amd as C
     if not torch.cuda.is_available():
         raise RuntimeError("bench.py needs a HIP device")
+    if args.config == "batch2d":
+        print(json.dumps(run_batch(args, torch)), flush=True)
+        return
     dev = torch.device("cuda:0")
     W = build_workload(args.config, args.lmax)
     N, dim, iters = W["ngbd"].num_nodes, W["dim"], W["iters"]
@@ -194,6 +332,15 @@ def main():
     rate_mixed = N * iters * args.steps / t_mixed
     rate_fp64 = N * iters * args.steps / t_fp64
     err = W["ngbd"].device().band_linf(out_mixed, out_fp64, out_fp64, 3 * h)
+    # host-buffer call (what a drop-in behind a host Vec sees): H2D + reinit + D2H
+    host_phi = W["phi0"].copy()
+    lsr.reinitialize_2nd_order(host_phi, iters, precision="mixed")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    host_phi[:] = W["phi0"]
+    lsr.reinitialize_2nd_order(host_phi, iters, precision="mixed")
+    t_host = time.perf_counter() - t0
+    rate_host = N * iters / t_host
     log(f"[bench] mixed {rate_mixed:.3e} node-iters/s, fp64 {rate_fp64:.3e}, x{rate_mixed / rate_fp64:.2f}, "
         f"Linf band err {err:.2e} = {err / h:.2e} h")
 
@@ -246,6 +393,8 @@ def main():
         "config": {"workload": W["desc"], "nodes": N, "iterations": iters, "h_min": h, "path": args.path},
         "fp64": {"value": rate_fp64, "unit": "node-iters/s", "ms_per_step": 1e3 * t_fp64 / args.steps},
         "mixed_over_fp64": rate_mixed / rate_fp64,
+        "pcie_inclusive": {"value": rate_host, "unit": "node-iters/s", "ms_per_step": 1e3 * t_host,
+                           "note": "phi in pageable host memory: copy in, reinitialize, copy out"},
         "linf_err_band": err,
         "linf_err_band_over_h": err / h,
         "roofline": roofline,

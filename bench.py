@@ -220,6 +220,100 @@ def run_batch(args, torch):
     }
 
 
+def run_curvature(args, torch):
+    """Learned h*kappa: about 1e6 training stencils from circles and sinusoids
+    on h = 2^-7 patches (reinitialized on the GPU), 10-128x4-1 BF16 network
+    trained with Adam at batch 16384; mean |h kappa| error of the hybrid
+    estimate against numerical curvature on held-out shapes, and the GPU time
+    per sample of both."""
+    from casl_p4est_amd import curvature as CV
+    patch = CV.UniformPatch(level=8, size=2.0)
+    t0 = time.time()
+    tr = CV.make_dataset(patch, 1500, 460, seed=SEED, log=log)
+    te = CV.make_dataset(patch, 150, 50, seed=SEED + 1)
+    log(f"[bench] curvature data: {len(tr['target'])} train / {len(te['target'])} test samples in "
+        f"{time.time() - t0:.1f}s (reinit {tr['reinit_s']:.1f}s, sampling {tr['samples_s']:.2f}s)")
+    net = CV.CurvatureNet(seed=SEED % 1000)
+    from casl_p4est_amd import _lib as L
+    import ctypes
+    lib = L.lib()
+    lib.casl_profile_enable(1)
+    lib.casl_profile_reset()
+    hist = CV.train(net, tr["feat"], tr["target"], epochs=args.epochs, batch=16384, lr=2e-3, log=log)
+    ms, cnt = ctypes.c_double(), ctypes.c_int64()
+    L.call("casl_profile_query", b"k_gemm", ctypes.byref(ms), ctypes.byref(cnt))
+    lib.casl_profile_enable(0)
+    dims = net.dims
+    macs = sum(i * o for i, o in zip(dims[:-1], dims[1:]))
+    # forward + the two backward products per layer (the first layer's input gradient is skipped)
+    flops = 2 * 16384 * (3 * macs - dims[0] * dims[1]) * hist["steps"]
+    gemm_tflops = flops / (ms.value * 1e-3) / 1e12
+    # evaluation on held-out shapes
+    feat, t = te["feat"], te["target"]
+    hk = CV.hybrid_hk(net, feat).cpu().numpy().astype(np.float64)
+    ihk = feat[:, 9].cpu().numpy().astype(np.float64)
+    e_num, e_ml = float(np.abs(ihk - t).mean()), float(np.abs(hk - t).mean())
+    big = np.abs(ihk) >= CV.LO_MIN_HK
+    kinds = {}
+    for k, name in ((0, "circles"), (1, "sinusoids")):
+        m = te["kind"] == k
+        kinds[name] = dict(numerical=float(np.abs(ihk[m] - t[m]).mean()), learned=float(np.abs(hk[m] - t[m]).mean()),
+                           samples=int(m.sum()))
+    # GPU cost per sample: numerical pipeline vs numerical + network
+    phis = torch.empty((64, patch.n_nodes), dtype=torch.float64, device="cuda")
+    rng = np.random.default_rng(SEED + 2)
+    for k in range(64):
+        phis[k] = torch.from_numpy(CV.circle_phi0(patch.xy, (1.0, 1.0), rng.uniform(0.05, 0.5)))
+    patch.reinitialize(phis, 20)
+    f, _, _ = patch.samples(phis)
+    reps = 20
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    for _ in range(reps):
+        f, _, _ = patch.samples(phis)
+    torch.cuda.synchronize()
+    t_num = (time.perf_counter() - a) / reps
+    a = time.perf_counter()
+    for _ in range(reps):
+        CV.hybrid_hk(net, f)
+    torch.cuda.synchronize()
+    t_ml = (time.perf_counter() - a) / reps
+    n_s = f.shape[0]
+    log(f"[bench] curvature: mean |hk err| numerical {e_num:.3e}, learned {e_ml:.3e} (x{e_num / e_ml:.2f}); "
+        f"training {hist['seconds']:.1f}s, GEMM {gemm_tflops:.1f} TFLOP/s; per sample numerical "
+        f"{1e9 * t_num / n_s:.1f} ns, network {1e9 * t_ml / n_s:.1f} ns")
+    return {
+        "metric": "learned h*kappa mean |error| on held-out circles and sinusoids (lower is better)",
+        "value": e_ml,
+        "unit": "|h kappa| error",
+        "n_gpus": 1,
+        "steps": hist["steps"],
+        "warmup": 0,
+        "ms_per_step": 1e3 * hist["seconds"] / max(1, hist["steps"]),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 mfma, fp32 accumulate",
+        "data": "synthetic",
+        "config": {"workload": "h = 2^-7 uniform patches, circles R U[1.6h,0.5] and sinusoids A U[0.01,0.2] "
+                               "omega U[2pi,12pi], 20 FP64 reinit iterations, 10-128x4-1 ReLU network, Adam, "
+                               "batch 16384", "train_samples": int(len(tr["target"])),
+                   "test_samples": int(len(t)), "epochs": args.epochs},
+        "numerical_error": e_num,
+        "learned_over_numerical": e_ml / e_num,
+        "by_shape": kinds,
+        "error_where_network_used": {"numerical": float(np.abs(ihk[big] - t[big]).mean()),
+                                     "learned": float(np.abs(hk[big] - t[big]).mean()), "samples": int(big.sum())},
+        "training": {"seconds": hist["seconds"], "rows_per_s": hist["rows"] / hist["seconds"],
+                     "final_loss_unit_variance": hist["loss"][-1]},
+        "roofline": {"bound": "mfma", "kernel": "k_gemm", "achieved": gemm_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": gemm_tflops / 2500.0, "traffic": None,
+                     "note": "training GEMMs are 128 wide: far from the dense BF16 peak"},
+        "gpu_ns_per_sample": {"numerical": 1e9 * t_num / n_s, "network": 1e9 * t_ml / n_s},
+        "cpu_baseline": None,
+    }
+
+
 def time_steps(fn, steps, warmup, torch):
     for _ in range(warmup):
         fn()
@@ -290,7 +384,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d", "batch2d"])
+    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d", "batch2d", "curvature2d"])
+    ap.add_argument("--epochs", type=int, default=20, help="training epochs (curvature2d)")
     ap.add_argument("--lmax", type=int, default=0, help="reduce max level (quick runs only)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -308,6 +403,9 @@ def main():
         raise RuntimeError("bench.py needs a HIP device")
     if args.config == "batch2d":
         print(json.dumps(run_batch(args, torch)), flush=True)
+        return
+    if args.config == "curvature2d":
+        print(json.dumps(run_curvature(args, torch)), flush=True)
         return
     dev = torch.device("cuda:0")
     W = build_workload(args.config, args.lmax)

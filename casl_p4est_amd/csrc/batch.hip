@@ -84,7 +84,7 @@ __global__ void __launch_bounds__(BT) k_batch(BatchArgs<D, R> A) {
     for_nodes<BT, NPT>(b, e, [&](int64_t n) { node_dd<D, double>(t, n, A.phi0, A.dd0); });
     __syncthreads();
     for_nodes<BT, NPT>(b, e, [&](int64_t n) {
-      node_prep<D, R>(t, n, A.phi0, A.dd0, nullptr, -DBL_MAX, DBL_MAX, A.zdt, true, A.flags, A.sfix);
+      node_prep<D, R>(t, n, n, A.phi0, A.dd0, nullptr, -DBL_MAX, DBL_MAX, A.zdt, true, A.flags, A.sfix);
     });
     __syncthreads();
     for (int it = 0; it < A.n_iter; ++it) {

@@ -16,6 +16,8 @@
 // cancel -- and the band error reduction in FP64.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
+
 #include <atomic>
 #include <cstring>
 #include <map>
@@ -46,10 +48,11 @@ template <int D, class S>
 __global__ void __launch_bounds__(BLOCK)
     k_prep(Tables<D> t, const double *__restrict__ phi0, const double *__restrict__ dd0,
            const uint8_t *__restrict__ mask, double lim_low, double lim_high, double zdt, int second_order,
-           uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
+           const int32_t *__restrict__ pos, uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
-  node_prep<D, S>(t, n, phi0, dd0, mask, lim_low, lim_high, zdt, second_order != 0, flags, s_fix);
+  const int64_t m = pos ? int64_t(pos[n]) : n;
+  node_prep<D, S>(t, n, m, phi0, dd0, mask, lim_low, lim_high, zdt, second_order != 0, flags, s_fix);
 }
 
 // Pass B of one pseudo-time stage.  out[n] = result, or for the second RK2
@@ -146,9 +149,12 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
     hipLaunchKernelGGL(k_d2f, dim3(grid), dim3(BLOCK), 0, s, phi, st->phif.p, N);
     HIP_CHECK(hipGetLastError());
   }
+  // the tiled stage kernel reads flags and s_fix in its record order
+  const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
+  const int32_t *pos = tiled ? st->t_pos.p : nullptr;
   prof_begin("k_prep", s);
   hipLaunchKernelGGL((k_prep<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p, mask, lim_low,
-                     lim_high, st->zdt, int(so), st->flags.p, sfix);
+                     lim_high, st->zdt, int(so), pos, st->flags.p, sfix);
   prof_end("k_prep", s);
   HIP_CHECK(hipGetLastError());
 
@@ -179,7 +185,6 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
     stage_launch(in, out, avg);
     prof_end("k_stage", s);
   };
-  const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
   TileArgs<R> ta{};
   if (tiled) {
     ta.tile_start = st->t_tile_start.p;
@@ -190,9 +195,14 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
     ta.rec = reinterpret_cast<const TileRec *>(st->t_rec.p);
     ta.rec_slot = st->t_rec_slot.p;
     ta.ext = st->t_ext.p;
+    ta.geom.pow2 = 1;
     for (int a = 0; a < 3; ++a) {
       ta.geom.base[a] = R(st->geom_base[a]);
       ta.geom.unit[a] = R(st->geom_unit[a]);
+      int e = 0;
+      const double m = std::frexp(st->geom_base[a], &e);
+      ta.geom.inv_base[a] = R(1.0 / st->geom_base[a]);
+      if (m != 0.5) ta.geom.pow2 = 0;
     }
     ta.flags = st->flags.p;
     ta.sfix = sfix;
@@ -326,6 +336,18 @@ void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
   st->t_rec.ensure(H.rec.size());
   HIP_CHECK(hipMemcpy(st->t_rec.p, H.rec.data(), H.rec.size() * sizeof(TileRec), hipMemcpyHostToDevice));
   up(st->t_ext, H.ext.empty() ? std::vector<uint16_t>(8, 0) : H.ext);
+  // node -> position of its record among the tile records (tile start + rank)
+  {
+    const int64_t N = tab->n_nodes;
+    std::vector<int32_t> pos(size_t(N), -1);
+    for (int64_t tt = 0; tt + 1 < int64_t(H.tile_start.size()); ++tt) {
+      const int32_t s0 = H.tile_start[tt], nT = H.tile_start[tt + 1] - s0, rb = H.rec_start[tt];
+      for (int32_t i = 0; i < nT; ++i) pos[size_t(s0 + H.rec_slot[size_t(rb + i)])] = s0 + i;
+    }
+    for (int64_t n = 0; n < N; ++n)
+      if (pos[size_t(n)] < 0) throw std::runtime_error("tile records do not cover every node once");
+    up(st->t_pos, pos);
+  }
   st->n_tiles = int64_t(H.tile_start.size()) - 1;
   st->n_halo = int64_t(H.halo_idx.size());
   st->n_records = int64_t(H.rec.size());

@@ -112,24 +112,25 @@ __device__ __forceinline__ void node_dd(const Tables<D> &t, int64_t n, const R *
 
 // The phi_0-only part of reinitialize_one_iteration (src/my_p4est_level_set.cpp:82-211):
 // mode, sign of phi_0 and the subcell arm lengths of crossing arms,
-// s_fix[f][n] = MAX(interface_Location*(..), zero_distance_threshold).
-// The reference recomputes these identically at every stage.
+// s_fix[f][m] = MAX(interface_Location*(..), zero_distance_threshold).
+// The reference recomputes these identically at every stage.  m is where the
+// node's entries go: n itself, or its position in the tiled record order.
 template <int D, class S>
-__device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, const double *__restrict__ phi0,
+__device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, int64_t m, const double *__restrict__ phi0,
                                           const double *__restrict__ dd0, const uint8_t *__restrict__ mask,
                                           double lim_low, double lim_high, double zdt, bool second_order,
                                           uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
   const double p = phi0[n];
   if (mask && mask[n] == 0) {
-    flags[n] = MODE_MASKED;
+    flags[m] = MODE_MASKED;
     return;
   }
   if (fabs(p) < zdt) {
-    flags[n] = MODE_ANCHOR;
+    flags[m] = MODE_ANCHOR;
     return;
   }
   if (!(lim_low < p && p < lim_high)) {
-    flags[n] = MODE_COPY0;
+    flags[m] = MODE_COPY0;
     return;
   }
   constexpr int F = 2 * D;
@@ -159,11 +160,11 @@ __device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, const d
         else
           s = interface_location(0.0, s, p0_000, p0_f[f]);
       }
-      s_fix[int64_t(f) * t.N + n] = S(max2(s, zdt));
+      s_fix[int64_t(f) * t.N + m] = S(max2(s, zdt));
     }
     fl |= uint16_t(cross << 3);
   }
-  flags[n] = fl;
+  flags[m] = fl;
 }
 
 // Pass B at one node: the stage result of reinitialize_one_iteration
@@ -231,10 +232,32 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
   }
   // one-sided differences with ENO/minmod correction (:230-248)
   R Dp[D], Dm[D];
+  bool divided = false;
+  if constexpr (TB::FASTDIV) {
+    if (t.fast) {
+      // no wall arm: every non-crossing s[f] is the power-of-two arm length,
+      // so dividing by it is multiplying by its exact inverse
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    Dp[a] = (pn_f[2 * a + 1] - pn_000) / s[2 * a + 1];
-    Dm[a] = (pn_000 - pn_f[2 * a]) / s[2 * a];
+      for (int a = 0; a < D; ++a) {
+        Dp[a] = (pn_f[2 * a + 1] - pn_000) * t.iq[2 * a + 1];
+        Dm[a] = (pn_000 - pn_f[2 * a]) * t.iq[2 * a];
+      }
+      if (cross) {
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+          if (cross & (2u << (2 * a))) Dp[a] = (pn_f[2 * a + 1] - pn_000) / s[2 * a + 1];
+          if (cross & (1u << (2 * a))) Dm[a] = (pn_000 - pn_f[2 * a]) / s[2 * a];
+        }
+      }
+      divided = true;
+    }
+  }
+  if (!divided) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+      Dp[a] = (pn_f[2 * a + 1] - pn_000) / s[2 * a + 1];
+      Dm[a] = (pn_000 - pn_f[2 * a]) / s[2 * a];
+    }
   }
   if constexpr (SECOND_ORDER) {
 #pragma unroll
@@ -295,7 +318,7 @@ struct casl_stencil {
   int64_t n_tiles = 0, n_halo = 0, n_records = 0, n_irregular = 0;
   double geom_base[3] = {0, 0, 0}, geom_unit[3] = {0, 0, 0};
   std::string tile_error;
-  casl::DevBuf<int32_t> t_tile_start, t_halo_start, t_h1count, t_halo, t_rec_start;
+  casl::DevBuf<int32_t> t_tile_start, t_halo_start, t_h1count, t_halo, t_rec_start, t_pos;
   casl::DevBuf<uint16_t> t_rec_slot;
   casl::DevBuf<uint4> t_rec;
   casl::DevBuf<uint16_t> t_ext;

@@ -19,6 +19,7 @@ constexpr double EPS = 1e-13;  // src/casl_math.h:23
 template <int D>
 struct Tables {
   static constexpr bool REGULAR = false;  // see the REGULAR shortcuts below
+  static constexpr bool FASTDIV = false;  // see second_difference
   static constexpr int F = 2 * D;
   static constexpr int C = 1 << (D - 1);
   static constexpr int T = 2 * (D - 1);
@@ -73,6 +74,37 @@ __device__ __forceinline__ double interface_location_2nd(double a, double b, dou
   return x + 0.5 * (a + b);
 }
 
+// Exact-inverse division.  A stencil accessor with TB::FASTDIV and t.fast
+// has power-of-two face distances d_f = 1 / iq[f] (cubic dyadic cells of a
+// power-of-two tree, no wall arm).  Dividing by a power of two and
+// multiplying by its exactly representable inverse give the same correctly
+// rounded result, so these helpers return bit-for-bit what the plain
+// division returns, without its instruction sequence.
+// The helpers below serve the hanging-arm arithmetic, where the test on
+// t.fast is a per-lane branch: it pays off against FP64 division sequences
+// and costs more than FP32's short approximate division, so they apply the
+// exact inverses in FP64 only.
+template <class R, class TB>
+constexpr bool irregular_fastdiv() {
+  return TB::FASTDIV && sizeof(R) == 8;
+}
+
+template <class R, class TB>
+__device__ __forceinline__ R div_d(const TB &t, R x, int f, int64_t n) {
+  if constexpr (irregular_fastdiv<R, TB>()) {
+    if (t.fast) return x * t.iq[f];
+  }
+  return x / R(t.d(f, n));
+}
+// x / (d_{2a+1} + d_{2a}): a power of two when both arms are equal
+template <class R, class TB>
+__device__ __forceinline__ R div_dsum(const TB &t, R x, int a, int64_t n) {
+  if constexpr (irregular_fastdiv<R, TB>()) {
+    if (t.fast && t.dist[2 * a + 1] == t.dist[2 * a]) return x * (R(0.5) * t.iq[2 * a + 1]);
+  }
+  return x / (R(t.d(2 * a + 1, n)) + R(t.d(2 * a, n)));
+}
+
 // get_linear_interpolation_weights + interpolate_linearly,
 // my_p4est_quad_neighbor_nodes_of_node.h:222-306
 // TB is any stencil accessor with idx(f, c, n), d(f, n) and dt(f, k, n):
@@ -94,7 +126,12 @@ __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G 
     const R w_mm = d_p0;
     const R w_pm = d_m0;
     const R normalization = (d_m0 + d_p0);
-    return (R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 0, n)]) * w_mm) / normalization;
+    const R num = R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 0, n)]) * w_mm;
+    // d_m0 + d_p0 is the arm cell's transverse size, which equals d_f here
+    if constexpr (irregular_fastdiv<R, TB>()) {
+      if (t.fast) return num * t.iq[f];
+    }
+    return num / normalization;
   } else {
     const R d_0m = R(t.dt(f, 2, n)), d_0p = R(t.dt(f, 3, n));
     const R w_mm = d_p0 * d_0p;
@@ -102,9 +139,13 @@ __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G 
     const R w_mp = d_p0 * d_0m;
     const R w_pp = d_m0 * d_0m;
     const R normalization = (d_m0 + d_p0) * (d_0m + d_0p);
-    return (R(g[t.idx(f, 3, n)]) * w_pp + R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 2, n)]) * w_mp +
-            R(g[t.idx(f, 0, n)]) * w_mm) /
-           normalization;
+    const R num = R(g[t.idx(f, 3, n)]) * w_pp + R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 2, n)]) * w_mp +
+                  R(g[t.idx(f, 0, n)]) * w_mm;
+    // both transverse sums are the arm cell's size d_f (cubic cells)
+    if constexpr (irregular_fastdiv<R, TB>()) {
+      if (t.fast) return num * (t.iq[f] * t.iq[f]);
+    }
+    return num / normalization;
   }
 }
 
@@ -114,6 +155,23 @@ __device__ __forceinline__ R central_second_derivative(R fp, R f0, R fm, R dp, R
   return ((fp - f0) / dp + (fm - f0) / dm) * R(2.) / (dp + dm);
 }
 
+// central_second_derivative along axis a of a stencil.  When the accessor
+// knows its face distances are powers of two (TB::FASTDIV and t.fast), the
+// divisions by d_p and d_m become multiplications by their exact inverses and
+// the division by d_p + d_m one by 1/(2 d_p) when d_p == d_m: the same
+// correctly rounded results, without the division sequences.
+template <class R, class TB>
+__device__ __forceinline__ R second_difference(const TB &t, int a, R fp, R f0, R fm, int64_t n) {
+  const int fpp = 2 * a + 1, fmm = 2 * a;
+  if constexpr (TB::FASTDIV) {
+    if (t.fast) {
+      const R x = ((fp - f0) * t.iq[fpp] + (fm - f0) * t.iq[fmm]) * R(2.);
+      return t.dist[fpp] == t.dist[fmm] ? x * (R(0.5) * t.iq[fpp]) : x / (t.dist[fpp] + t.dist[fmm]);
+    }
+  }
+  return central_second_derivative(fp, f0, fm, R(t.d(fpp, n)), R(t.d(fmm, n)));
+}
+
 template <class R>
 __device__ __forceinline__ bool check_if_zero(R v) { return fabs(v) < R(EPS); }
 
@@ -121,11 +179,10 @@ __device__ __forceinline__ bool check_if_zero(R v) { return fabs(v) < R(EPS); }
 template <int D, class R, class TB>
 __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R &Dyy, R &Dzz) {
   if constexpr (TB::REGULAR) return;
-  const R d_m00 = R(t.d(0, n)), d_p00 = R(t.d(1, n)), d_0m0 = R(t.d(2, n)), d_0p0 = R(t.d(3, n));
-  const R m12 = R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)) / d_m00 / (d_p00 + d_m00) +
-                R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)) / d_p00 / (d_p00 + d_m00);
-  const R m21 = R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)) / d_0m0 / (d_0p0 + d_0m0) +
-                R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)) / d_0p0 / (d_0p0 + d_0m0);
+  const R m12 = div_dsum<R>(t, div_d<R>(t, R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)), 0, n), 0, n) +
+                div_dsum<R>(t, div_d<R>(t, R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)), 1, n), 0, n);
+  const R m21 = div_dsum<R>(t, div_d<R>(t, R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)), 2, n), 1, n) +
+                div_dsum<R>(t, div_d<R>(t, R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)), 3, n), 1, n);
   if constexpr (D == 2) {
     if (check_if_zero(m12)) {
       if (!check_if_zero(m21)) Dyy -= m21 * Dxx;
@@ -138,15 +195,14 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
       Dyy = nDyy * idet - nDxx * (m21 * idet);
     }
   } else {
-    const R d_00m = R(t.d(4, n)), d_00p = R(t.d(5, n));
-    const R m13 = R(t.dt(0, 2, n)) * R(t.dt(0, 3, n)) / d_m00 / (d_p00 + d_m00) +
-                  R(t.dt(1, 2, n)) * R(t.dt(1, 3, n)) / d_p00 / (d_p00 + d_m00);
-    const R m23 = R(t.dt(2, 2, n)) * R(t.dt(2, 3, n)) / d_0m0 / (d_0p0 + d_0m0) +
-                  R(t.dt(3, 2, n)) * R(t.dt(3, 3, n)) / d_0p0 / (d_0p0 + d_0m0);
-    const R m31 = R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)) / d_00m / (d_00p + d_00m) +
-                  R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)) / d_00p / (d_00p + d_00m);
-    const R m32 = R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)) / d_00m / (d_00p + d_00m) +
-                  R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)) / d_00p / (d_00p + d_00m);
+    const R m13 = div_dsum<R>(t, div_d<R>(t, R(t.dt(0, 2, n)) * R(t.dt(0, 3, n)), 0, n), 0, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(1, 2, n)) * R(t.dt(1, 3, n)), 1, n), 0, n);
+    const R m23 = div_dsum<R>(t, div_d<R>(t, R(t.dt(2, 2, n)) * R(t.dt(2, 3, n)), 2, n), 1, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(3, 2, n)) * R(t.dt(3, 3, n)), 3, n), 1, n);
+    const R m31 = div_dsum<R>(t, div_d<R>(t, R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)), 4, n), 2, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)), 5, n), 2, n);
+    const R m32 = div_dsum<R>(t, div_d<R>(t, R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)), 4, n), 2, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)), 5, n), 2, n);
     if (check_if_zero(m12) && check_if_zero(m13)) {
       if (check_if_zero(m23)) {
         const bool z21 = !check_if_zero(m21), z31 = !check_if_zero(m31), z32 = !check_if_zero(m32);
@@ -221,10 +277,10 @@ __device__ __forceinline__ void laplace(const TB &t, int64_t n, const G *__restr
   f000 = R(g[n]);
 #pragma unroll
   for (int f = 0; f < 2 * D; ++f) fnb[f] = face_linear<D, R>(t, f, n, g);
-  dd[0] = central_second_derivative(fnb[1], f000, fnb[0], R(t.d(1, n)), R(t.d(0, n)));
-  dd[1] = central_second_derivative(fnb[3], f000, fnb[2], R(t.d(3, n)), R(t.d(2, n)));
+  dd[0] = second_difference<R>(t, 0, fnb[1], f000, fnb[0], n);
+  dd[1] = second_difference<R>(t, 1, fnb[3], f000, fnb[2], n);
   R dzz = R(0);
-  if constexpr (D == 3) dzz = central_second_derivative(fnb[5], f000, fnb[4], R(t.d(5, n)), R(t.d(4, n)));
+  if constexpr (D == 3) dzz = second_difference<R>(t, 2, fnb[5], f000, fnb[4], n);
   correct_naive<D, R>(t, n, dd[0], dd[1], dzz);
   if constexpr (D == 3) dd[2] = dzz;
 }

@@ -12,20 +12,29 @@ namespace dev {
 
 template <class R>
 struct TileGeom {
-  R base[3];  // coarsest arm-cell size per axis
-  R unit[3];  // finest cell size per axis
+  R base[3];      // coarsest arm-cell size per axis
+  R unit[3];      // finest cell size per axis
+  R inv_base[3];  // 1 / base, exact when pow2
+  int pow2;       // every base[a] is a power of two: all arm lengths are, too
 };
 
 __device__ __forceinline__ double pow2neg(double, int l) { return __longlong_as_double((1023ll - l) << 52); }
 __device__ __forceinline__ float pow2neg(float, int l) { return __int_as_float((127 - l) << 23); }
+__device__ __forceinline__ double pow2pos(double, int l) { return __longlong_as_double((1023ll + l) << 52); }
+__device__ __forceinline__ float pow2pos(float, int l) { return __int_as_float((127 + l) << 23); }
 
 // Face distances of a record: arm-cell sizes from the level nibbles, wall arms
 // -(d_opposite) - (second-degree cell size) (construct_neighbors :609-719).
+// iq[f] = 1 / q[f], exact when the geometry is pow2.
 template <int D, class R>
-__device__ __forceinline__ void decode_dist(uint32_t geo, const TileGeom<R> &g, R *dist, R *q) {
+__device__ __forceinline__ void decode_dist(uint32_t geo, const TileGeom<R> &g, R *dist, R *q, R *iq) {
   constexpr int F = 2 * D;
 #pragma unroll
-  for (int f = 0; f < F; ++f) q[f] = g.base[f >> 1] * pow2neg(R(0), int((geo >> (4 * f)) & 15u));
+  for (int f = 0; f < F; ++f) {
+    const int l = int((geo >> (4 * f)) & 15u);
+    q[f] = g.base[f >> 1] * pow2neg(R(0), l);
+    iq[f] = g.inv_base[f >> 1] * pow2pos(R(0), l);
+  }
 #pragma unroll
   for (int f = 0; f < F; ++f) dist[f] = q[f];
 #pragma unroll
@@ -33,47 +42,60 @@ __device__ __forceinline__ void decode_dist(uint32_t geo, const TileGeom<R> &g, 
     if ((geo >> (24 + f)) & 1u) dist[f] = -dist[f ^ 1] - q[f];
 }
 
+// Face distances shared by both stencil kinds.  fast: pow2 geometry and no
+// wall arm, so every face distance d is a power of two and x / d == x * iq
+// exactly (both are the correctly rounded value of the same real number);
+// FASTDIV lets the stencil arithmetic use that.
+template <int D, class R>
+struct DecodedDist {
+  static constexpr bool FASTDIV = true;
+  static constexpr int F = 2 * D;
+  R dist[F];
+  R q[F];
+  R iq[F];
+  bool fast;
+  __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
+  __device__ __forceinline__ void decode(uint32_t geo, const TileGeom<R> &g) {
+    decode_dist<D, R>(geo, g, dist, q, iq);
+    fast = g.pow2 && ((geo >> 24) & 63u) == 0u;
+  }
+};
+
 // Stencil of a regular node (no hanging arm): one LDS-local corner per face.
 template <int D, class R>
-struct RegStencil {
+struct RegStencil : DecodedDist<D, R> {
   static constexpr bool REGULAR = true;
   static constexpr int F = 2 * D;
   uint16_t id[F];
-  R dist[F];
-  R q[F];
   __device__ __forceinline__ int32_t idx(int f, int, int64_t) const { return id[f]; }
-  __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
   // transverse offsets (0, arm cell) -- only their product (0) enters the math
-  __device__ __forceinline__ R dt(int f, int k, int64_t) const { return (k & 1) ? q[f] : R(0); }
+  __device__ __forceinline__ R dt(int f, int k, int64_t) const { return (k & 1) ? this->q[f] : R(0); }
   __device__ __forceinline__ void load(const TileRec &r, const TileGeom<R> &g) {
 #pragma unroll
     for (int f = 0; f < F; ++f) id[f] = r.idx[f];
-    decode_dist<D, R>(r.geo, g, dist, q);
+    this->decode(r.geo, g);
   }
 };
 
 // Stencil of a node with a hanging arm: corners and transverse offsets are
 // read from its extension record when the math asks for them.
 template <int D, class R>
-struct IrrStencil {
+struct IrrStencil : DecodedDist<D, R> {
   static constexpr bool REGULAR = false;
   static constexpr int F = 2 * D, C = 1 << (D - 1);
   static constexpr int W = (D == 2 ? 3 : 6);  // uint16 words per face extension
   const uint16_t *__restrict__ e;
-  R dist[F];
-  R q[F];
   R unit;
   __device__ __forceinline__ int32_t idx(int f, int c, int64_t) const { return e[f * W + c]; }
-  __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
   __device__ __forceinline__ R dt(int f, int k, int64_t) const {
     const R o = R(e[f * W + C + (k >> 1)]) * unit;  // m0 (k = 0, 1) or 0m (k = 2, 3)
-    return (k & 1) ? q[f] - o : o;
+    return (k & 1) ? this->q[f] - o : o;
   }
   __device__ __forceinline__ void load(const TileRec &r, const uint16_t *__restrict__ ext, const TileGeom<R> &g) {
     const uint32_t id = uint32_t(r.idx[0]) | (uint32_t(r.idx[1]) << 16);
     e = ext + size_t(id) * (F * W);
     unit = g.unit[0];
-    decode_dist<D, R>(r.geo, g, dist, q);
+    this->decode(r.geo, g);
   }
 };
 
@@ -110,8 +132,14 @@ constexpr int TILE_BLOCK = 256;
 
 // One pseudo-time stage of a tile: phi of T u H1 u H2 into LDS, pass A for
 // T u H1 in LDS, pass B for T.  out = result, or (RK_AVG) out = (out + result)/2.
-// Occupancy: the kernel is latency bound (gathers, two barriers per tile),
-// so the register budget is capped to keep TILE_WAVES_* waves per SIMD.
+//
+// Latency: every global load a node's work needs is issued before the first
+// barrier -- each thread's tile records (its own nodes, used in both passes),
+// their LDS slots, their flags (stored in record order by the prep pass, so
+// the address does not wait for the slot) and, for the second RK2 stage, the
+// phi_n value being averaged -- so after the phi gather the passes run on
+// registers and LDS (crossing arms aside, whose s_fix is read in pass B).
+// Occupancy: the register budget is capped to keep TILE_WAVES_* waves per SIMD.
 #ifndef CASL_TILE_WAVES_F32
 #define CASL_TILE_WAVES_F32 4
 #endif
@@ -123,6 +151,8 @@ __global__ void __launch_bounds__(TILE_BLOCK)
     __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
     k_tile_stage(TileArgs<R> A) {
   constexpr int LMAX = TileLimits<D>::LMAX, DMAX = TileLimits<D>::DMAX;
+  constexpr int KT = TileLimits<D>::TMAX / TILE_BLOCK;  // tile nodes per thread, at most
+  static_assert(TileLimits<D>::TMAX % TILE_BLOCK == 0, "tile size must be a multiple of the block");
   __shared__ R phis[LMAX];
   __shared__ R dds[SECOND_ORDER ? D * DMAX : 1];
   const int tile = xcd_tile(int(blockIdx.x), int(gridDim.x));
@@ -130,13 +160,29 @@ __global__ void __launch_bounds__(TILE_BLOCK)
   const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
   const int nH1 = A.h1count[tile];
   const int rb = A.rec_start[tile];
+  TileRec trec[KT];
+  int tslot[KT];
+  uint16_t tfl[KT];
+  R tprev[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    if (i < nT) {
+      trec[k] = A.rec[rb + i];
+      tslot[k] = A.rec_slot[rb + i];
+      tfl[k] = A.flags[s + i];
+    }
+  }
   for (int j = threadIdx.x; j < nT; j += TILE_BLOCK) phis[j] = A.phin[s + j];
   for (int j = threadIdx.x; j < nH; j += TILE_BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
+  if constexpr (RK_AVG) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (int(threadIdx.x) + k * TILE_BLOCK < nT) tprev[k] = A.out[s + tslot[k]];
+  }
   __syncthreads();
   if constexpr (SECOND_ORDER) {
-    for (int i = threadIdx.x; i < nT + nH1; i += TILE_BLOCK) {
-      const TileRec rec = A.rec[rb + i];
-      const int j = A.rec_slot[rb + i];
+    auto pass_a = [&](const TileRec &rec, int j) {
       R f000, fnb[2 * D], dd[D];
       if (!(rec.geo & GEO_IRREGULAR)) {
         RegStencil<D, R> st;
@@ -149,29 +195,36 @@ __global__ void __launch_bounds__(TILE_BLOCK)
       }
 #pragma unroll
       for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
-    }
+    };
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (int(threadIdx.x) + k * TILE_BLOCK < nT) pass_a(trec[k], tslot[k]);
+    for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
     __syncthreads();
   }
-  for (int i = threadIdx.x; i < nT; i += TILE_BLOCK) {
-    const TileRec rec = A.rec[rb + i];
-    const int j = A.rec_slot[rb + i];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    if (i >= nT) continue;
+    const TileRec &rec = trec[k];
+    const int j = tslot[k];
     const int64_t n = int64_t(s) + j;
-    const uint16_t fl = A.flags[n];
+    const uint16_t fl = tfl[k];
     const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
     R r;
     if (!(rec.geo & GEO_IRREGULAR)) {
       RegStencil<D, R> st;
       st.load(rec, A.geom);
-      r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + n, A.N, A.phi0 + n, phis,
-                                                                   dds, DMAX);
+      r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
+                                                                   phis, dds, DMAX);
     } else {
       IrrStencil<D, R> st;
       st.load(rec, A.ext, A.geom);
-      r = stage_value<D, R, SECOND_ORDER, IrrStencil<D, R>, true>(st, j, fl, w, A.sfix + n, A.N, A.phi0 + n, phis,
-                                                                   dds, DMAX);
+      r = stage_value<D, R, SECOND_ORDER, IrrStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
+                                                                   phis, dds, DMAX);
     }
     if constexpr (RK_AVG)
-      A.out[n] = R(0.5) * (A.out[n] + r);
+      A.out[n] = R(0.5) * (tprev[k] + r);
     else
       A.out[n] = r;
   }

@@ -51,16 +51,23 @@ struct FaceExt<3> {
 };
 
 // Tile sizing: at most TMAX tile nodes, LMAX nodes in T u H1 u H2 (LDS phi
-// slots) and DMAX nodes in T u H1 (LDS second-derivative slots).
+// slots) and DMAX nodes in T u H1 (LDS second-derivative slots).  The
+// defaults are build-time tunables (CASL_TILE_T2 / CASL_TILE_T3).
+#ifndef CASL_TILE_T2
+#define CASL_TILE_T2 1024
+#endif
+#ifndef CASL_TILE_T3
+#define CASL_TILE_T3 2048
+#endif
 template <int D>
 struct TileLimits;
 template <>
 struct TileLimits<2> {
-  static constexpr int TMAX = 1024, LMAX = 2048, DMAX = 1536;
+  static constexpr int TMAX = CASL_TILE_T2, LMAX = 2 * CASL_TILE_T2, DMAX = 3 * CASL_TILE_T2 / 2;
 };
 template <>
 struct TileLimits<3> {
-  static constexpr int TMAX = 512, LMAX = 2048, DMAX = 1024;
+  static constexpr int TMAX = CASL_TILE_T3, LMAX = 4 * CASL_TILE_T3, DMAX = 2 * CASL_TILE_T3;
 };
 
 struct TileHost {

@@ -128,7 +128,10 @@ struct TileArgs {
   int64_t N;
 };
 
-constexpr int TILE_BLOCK = 256;
+#ifndef CASL_TILE_BLOCK
+#define CASL_TILE_BLOCK 512
+#endif
+constexpr int TILE_BLOCK = CASL_TILE_BLOCK;
 
 // One pseudo-time stage of a tile: phi of T u H1 u H2 into LDS, pass A for
 // T u H1 in LDS, pass B for T.  out = result, or (RK_AVG) out = (out + result)/2.

@@ -55,59 +55,13 @@ def dd_bytes(dim, w):
 
 
 def build_workload(name, lmax_override=None):
-    import casl_p4est_amd as C
-    from casl_p4est_amd import levelset as ls
-    t0 = time.time()
-    if name == "spheres3d":
-        dim, lmin, lmax, iters = 3, 5, 10, 50
-        shape = ls.random_spheres(64, seed=SEED)
-        desc = ("3D octree unit cube, lmin 5 / lmax 10, |phi0|<=1.2 diag refinement, 64 spheres "
-                "(seed 20261202) x (1+0.5 sin 2pi x cos 2pi y), reinitialize_2nd_order 50 iterations")
-    elif name == "star2d":
-        dim, lmin, lmax, iters = 2, 4, 9, 20
-        shape = ls.Star()
-        desc = ("2D quadtree unit square, lmin 4 / lmax 9, star r=0.3+0.05cos5t x (1+0.8|x-c|^2), "
-                "reinitialize_2nd_order 20 iterations")
-    else:
-        raise ValueError(name)
-    if lmax_override:
-        lmax = lmax_override
-        desc += f" [lmax reduced to {lmax}]"
-    forest = C.Forest(dim).refine_levelset(shape, lmin, lmax, lip=2.4)
-    t1 = time.time()
-    forest.build_nodes()
-    t2 = time.time()
-    ngbd = C.NodeNeighbors(forest).init_neighbors()
-    t3 = time.time()
-    phi0 = forest.sample(shape)
-    log(f"[bench] host forest: refine {t1 - t0:.1f}s, nodes {t2 - t1:.1f}s, stencils {t3 - t2:.1f}s; "
-        f"{forest.num_nodes} nodes, {forest.num_leaves} leaves")
-    return dict(dim=dim, lmin=lmin, lmax=lmax, iters=iters, forest=forest, ngbd=ngbd, phi0=phi0, desc=desc)
+    from casl_p4est_amd import workloads
+    return workloads.build_tree(name, lmax_override, log=log)
 
 
 def build_batch_workload(n_trees=4096, threads=16):
-    """4096 independent quadtrees on the unit square: lmin 3, lmax U{6..9},
-    random star interfaces (centre U[0.35,0.65]^2, r0 U[0.15,0.3], amplitude
-    U[0,0.08], lobes U{3..8}) with phi0 scaled by (1 + 0.8 |x-c|^2)."""
-    from concurrent.futures import ThreadPoolExecutor
-    import casl_p4est_amd as C
-    from casl_p4est_amd import levelset as ls
-    rng = np.random.default_rng(SEED)
-    specs = [(ls.random_star(rng), int(rng.integers(6, 10))) for _ in range(n_trees)]
-
-    def one(spec):
-        shape, lmax = spec
-        f = C.Forest(2).refine_levelset(shape, 3, lmax, lip=2.4).build_nodes()
-        return C.flat_stencil_from_forest(f), f.sample(shape), float(np.min(f.dxyz_min()))
-
-    t0 = time.time()
-    with ThreadPoolExecutor(threads) as ex:
-        res = list(ex.map(one, specs))
-    log(f"[bench] host forests: {n_trees} trees in {time.time() - t0:.1f}s")
-    desc = (f"{n_trees} independent 2D quadtrees (unit square, lmin 3, lmax U{{6..9}}, random stars, "
-            f"phi0 x (1+0.8|x-c|^2), seed 20261202), reinitialize_2nd_order 20 iterations, one persistent launch")
-    return dict(tabs=[r[0] for r in res], phis=[r[1] for r in res], h=np.array([r[2] for r in res]),
-                lmax=np.array([s[1] for s in specs]), iters=20, desc=desc)
+    from casl_p4est_amd import workloads
+    return workloads.build_batch(n_trees, threads, log=log)
 
 
 def run_batch(args, torch):
@@ -384,7 +338,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d", "batch2d", "curvature2d"])
+    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d", "gyroid3d", "batch2d", "curvature2d"])
     ap.add_argument("--epochs", type=int, default=20, help="training epochs (curvature2d)")
     ap.add_argument("--lmax", type=int, default=0, help="reduce max level (quick runs only)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")

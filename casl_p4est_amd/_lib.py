@@ -98,6 +98,7 @@ SIGNATURES = {
     "casl_batch_get_schedule": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p]),
     "casl_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_void_p, c_int32_p, c_int32_p, ctypes.POINTER(ctypes.c_float)]),
+    "casl_batch_tile_info": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p]),
     "casl_batch_reinit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_void_p]),
     "casl_batch_reinit_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,

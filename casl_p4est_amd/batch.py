@@ -43,7 +43,13 @@ class TreeBatch:
             raise ValueError("phi must hold the concatenated node values of every tree")
 
     def set_schedule(self, block_threads, nodes_per_thread):
+        """nodes_per_thread 1 or 2: the grid-table kernel; 0: the tiled kernel."""
         L.call("casl_batch_set_schedule", self._h, int(block_threads), int(nodes_per_thread))
+
+    def tile_info(self):
+        t, n = ctypes.c_int32(), ctypes.c_int64()
+        L.call("casl_batch_tile_info", self._h, ctypes.byref(t), ctypes.byref(n))
+        return {"tiled": bool(t.value), "n_tiles": n.value}
 
     def schedule(self):
         b, n = ctypes.c_int32(), ctypes.c_int32()

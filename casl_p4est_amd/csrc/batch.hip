@@ -11,9 +11,14 @@
 // is the same device code as the grid-wide kernels, so each tree's result is
 // bit-identical to casl_reinit on that tree alone.
 //
-// Launch shape (threads per block x nodes per thread) is picked by
-// casl_batch_autotune; casl_batch_reinit_graph records the queue reset and
-// the persistent kernel into a hipGraph once and replays it.
+// Two kernels share that structure.  k_batch gathers through the grid-wide
+// tables, NPT nodes per thread.  k_batch_tiles runs every stage of its tree
+// as a sequence of the tree's LDS-staged tiles (the fused stage body of the
+// single-tree path, compact 16-byte records): each tree's tiled encoding is
+// built on the host, checked, and concatenated.  The launch shape -- kernel,
+// threads per block, nodes per thread -- is picked by casl_batch_autotune;
+// casl_batch_reinit_graph records the queue reset and the persistent kernel
+// into a hipGraph once and replays it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -22,7 +27,9 @@
 #include <numeric>
 #include <vector>
 
+#include "common.hpp"
 #include "reinit_device.hpp"
+#include "tiles_device.hpp"
 
 using namespace casl;
 using namespace casl::dev;
@@ -107,8 +114,90 @@ __global__ void __launch_bounds__(BT) k_batch(BatchArgs<D, R> A) {
   }
 }
 
+// Arguments of the tiled persistent kernel: the concatenated tiled encoding of
+// all trees (one geometry) and the per-tree tile ranges.
+template <class R>
+struct BatchTileArgs {
+  TileArgs<R> T;
+  const int64_t *__restrict__ off;
+  const int32_t *__restrict__ order;
+  const int32_t *__restrict__ tree_tile;  // [n_trees + 1]
+  unsigned int *__restrict__ queue;
+  double *__restrict__ phi;
+  R *__restrict__ cur;
+  R *__restrict__ nxt;
+  int32_t n_trees;
+  int n_iter;
+};
+
+// The phi_0-only prep of every batch node (grid-wide, before the persistent
+// kernel): phi_0 and the working copy, pass A on phi_0, flags and arm lengths
+// in tile-record order.
+template <class R>
+__global__ void k_batch_copy(const double *__restrict__ phi, double *__restrict__ phi0, R *__restrict__ cur,
+                             int64_t N) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  const double v = phi[n];
+  phi0[n] = v;
+  cur[n] = R(v);
+}
+template <int D>
+__global__ void k_batch_dd0(Tables<D> t, const double *__restrict__ phi0, double *__restrict__ dd0) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n < t.N) node_dd<D, double>(t, n, phi0, dd0);
+}
+template <int D, class R>
+__global__ void k_batch_prep(Tables<D> t, const double *__restrict__ phi0, const double *__restrict__ dd0,
+                             const int32_t *__restrict__ pos, double zdt, uint16_t *__restrict__ flags,
+                             R *__restrict__ sfix) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n < t.N) node_prep<D, R>(t, n, pos[n], phi0, dd0, nullptr, -DBL_MAX, DBL_MAX, zdt, true, flags, sfix);
+}
+
+// Persistent kernel over trees whose pseudo-time stages run tile by tile in
+// LDS; every tree's result is its single-tree tiled result.
+template <int D, class R, int BLOCK>
+__global__ void __launch_bounds__(BLOCK)
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
+    k_batch_tiles(BatchTileArgs<R> A) {
+  __shared__ R phis[TileLimits<D>::LMAX];
+  __shared__ R dds[D * TileLimits<D>::DMAX];
+  int32_t *s_tree = reinterpret_cast<int32_t *>(phis);  // between trees phis is free
+  for (;;) {
+    if (threadIdx.x == 0) {
+      const unsigned k = atomicAdd(A.queue, 1u);
+      *s_tree = (int32_t(k) < A.n_trees) ? A.order[k] : -1;
+    }
+    __syncthreads();
+    const int32_t tree = *s_tree;
+    __syncthreads();  // every thread has read the tree before the tiles overwrite phis
+    if (tree < 0) break;
+    const int t0 = A.tree_tile[tree], t1 = A.tree_tile[tree + 1];
+    TileArgs<R> a = A.T;
+    for (int it = 0; it < A.n_iter; ++it) {
+      a.phin = A.cur;
+      a.out = A.nxt;
+      for (int tile = t0; tile < t1; ++tile) {
+        tile_stage<D, R, true, false, BLOCK>(a, tile, phis, dds);
+        __syncthreads();
+      }
+      a.phin = A.nxt;
+      a.out = A.cur;
+      for (int tile = t0; tile < t1; ++tile) {
+        tile_stage<D, R, true, true, BLOCK>(a, tile, phis, dds);
+        __syncthreads();
+      }
+    }
+    const int64_t b = A.off[tree], e = A.off[tree + 1];
+    for (int64_t n = b + threadIdx.x; n < e; n += BLOCK) A.phi[n] = double(A.cur[n]);
+    __syncthreads();
+  }
+}
+
 __global__ void k_reset_queue(unsigned int *q) { *q = 0u; }
 
+// npt >= 1: k_batch with that many nodes per thread; npt == 0: k_batch_tiles.
 struct Schedule {
   int block = 256;
   int npt = 1;
@@ -119,6 +208,8 @@ const int kNpts[] = {1, 2};
 
 template <int D, class R>
 using KernelFn = void (*)(BatchArgs<D, R>);
+template <int D, class R>
+using TileKernelFn = void (*)(BatchTileArgs<R>);
 
 template <int D, class R>
 KernelFn<D, R> pick_kernel(const Schedule &s) {
@@ -127,15 +218,33 @@ KernelFn<D, R> pick_kernel(const Schedule &s) {
   CASL_PICK(64, 1) CASL_PICK(64, 2) CASL_PICK(128, 1) CASL_PICK(128, 2) CASL_PICK(256, 1) CASL_PICK(256, 2)
   CASL_PICK(512, 1) CASL_PICK(512, 2)
 #undef CASL_PICK
-  throw std::invalid_argument("unsupported batch schedule (block in {64,128,256,512}, nodes/thread in {1,2})");
+  throw std::invalid_argument("unsupported batch schedule (block in {64,128,256,512}, nodes/thread in {1,2}; "
+                              "nodes/thread 0: tiled kernel with block 256 (2D) or 512)");
+}
+
+// block sizes of the tiled kernel: a tile's nodes per thread stay small
+template <int D>
+constexpr bool tile_block_ok(int block) {
+  return block == 512 || (D == 2 && block == 256);
+}
+
+template <int D, class R>
+TileKernelFn<D, R> pick_tile_kernel(int block) {
+  if (block == 512) return &k_batch_tiles<D, R, 512>;
+  if constexpr (D == 2)
+    if (block == 256) return &k_batch_tiles<D, R, 256>;
+  throw std::invalid_argument("unsupported tiled batch block size");
 }
 
 }  // namespace
 
 struct casl_batch {
-  casl_stencil st;  // concatenated tables + scratch (phi0, dd0, flags)
+  casl_stencil st;  // concatenated tables + scratch (phi0, dd0, flags); tiled encodings of all trees
   int32_t n_trees = 0;
   int32_t max_tree = 0;
+  bool tiled = false;  // every tree has a tiled encoding (k_batch_tiles usable)
+  std::string tile_error;
+  casl::DevBuf<int32_t> tree_tile;  // [n_trees + 1] tile ranges
   casl::DevBuf<int64_t> off;
   casl::DevBuf<int32_t> order;
   casl::DevBuf<unsigned int> queue;
@@ -197,6 +306,57 @@ void launch_batch(casl_batch *B, double *phi, int n_iter, hipStream_t s, bool al
     A.ddn = st.ddnf.p;
   }
   if (allocate_only) return;
+  if (B->sched.npt == 0) {
+    if (!B->tiled) throw std::invalid_argument("the tiled batch kernel is not available: " + B->tile_error);
+    BatchTileArgs<R> P{};
+    TileArgs<R> &T = P.T;
+    T.tile_start = st.t_tile_start.p;
+    T.halo_start = st.t_halo_start.p;
+    T.h1count = st.t_h1count.p;
+    T.halo = st.t_halo.p;
+    T.rec_start = st.t_rec_start.p;
+    T.rec = reinterpret_cast<const TileRec *>(st.t_rec.p);
+    T.rec_slot = st.t_rec_slot.p;
+    T.ext = st.t_ext.p;
+    T.geom.pow2 = 1;
+    for (int a = 0; a < 3; ++a) {
+      T.geom.base[a] = R(st.geom_base[a]);
+      T.geom.unit[a] = R(st.geom_unit[a]);
+      T.geom.inv_base[a] = R(1.0 / st.geom_base[a]);
+      int ex = 0;
+      if (std::frexp(st.geom_base[a], &ex) != 0.5) T.geom.pow2 = 0;
+    }
+    T.flags = st.flags.p;
+    T.sfix = A.sfix;
+    T.phi0 = st.phi0.p;
+    T.N = N;
+    P.off = B->off.p;
+    P.order = B->order.p;
+    P.tree_tile = B->tree_tile.p;
+    P.queue = B->queue.p;
+    P.phi = phi;
+    P.cur = A.cur;
+    P.nxt = A.nxt;
+    P.n_trees = B->n_trees;
+    P.n_iter = n_iter;
+    const unsigned g = unsigned((N + 255) / 256);
+    hipLaunchKernelGGL((k_batch_copy<R>), dim3(g), dim3(256), 0, s, phi, st.phi0.p, A.cur, N);
+    hipLaunchKernelGGL((k_batch_dd0<D>), dim3(g), dim3(256), 0, s, A.t, st.phi0.p, st.dd0.p);
+    hipLaunchKernelGGL((k_batch_prep<D, R>), dim3(g), dim3(256), 0, s, A.t, st.phi0.p, st.dd0.p, st.t_pos.p,
+                       st.zdt, st.flags.p, A.sfix);
+    HIP_CHECK(hipGetLastError());
+    TileKernelFn<D, R> fn = pick_tile_kernel<D, R>(B->sched.block);
+    int occ = 0;
+    HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, B->sched.block, 0));
+    occ = std::max(occ, 1);
+    const int grid = std::max(1, std::min<int>(B->n_trees, B->cus * occ));
+    hipLaunchKernelGGL(k_reset_queue, dim3(1), dim3(1), 0, s, B->queue.p);
+    prof_begin("k_batch", s);
+    hipLaunchKernelGGL(fn, dim3(grid), dim3(B->sched.block), 0, s, P);
+    prof_end("k_batch", s);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   KernelFn<D, R> fn = pick_kernel<D, R>(B->sched);
   int occ = 0;
   HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void *)fn, B->sched.block, 0));
@@ -207,6 +367,93 @@ void launch_batch(casl_batch *B, double *phi, int n_iter, hipStream_t s, bool al
   hipLaunchKernelGGL(fn, dim3(grid), dim3(B->sched.block), 0, s, A);
   prof_end("k_batch", s);
   HIP_CHECK(hipGetLastError());
+}
+
+// Tiled encodings of every tree (built in parallel), checked one by one and
+// concatenated: node, halo, record and extension indices offset per tree.
+void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std::vector<int64_t> &off) {
+  const int32_t nt = B->n_trees;
+  const int D = trees[0].dim, F = 2 * D, W = (D == 2 ? 3 : 6);
+  std::vector<TileHost> hosts(static_cast<size_t>(nt));
+  std::vector<std::string> errs(static_cast<size_t>(nt));
+  parallel_for(nt, [&](int64_t b, int64_t e) {
+    for (int64_t k = b; k < e; ++k) {
+      try {
+        build_tiles(trees[k], hosts[size_t(k)]);
+        validate_tile_host(hosts[size_t(k)], D, trees[k].n_nodes);
+      } catch (const std::exception &ex) {
+        errs[size_t(k)] = ex.what();
+      }
+    }
+  }, 1);
+  for (int32_t k = 0; k < nt; ++k)
+    if (!errs[size_t(k)].empty()) {
+      B->tile_error = "tree " + std::to_string(k) + ": " + errs[size_t(k)];
+      return;
+    }
+  // one geometry for the batch: equal coarsest arm cells, extension offsets
+  // rescaled to the finest unit of all trees (exact: powers of two)
+  TileHost all;
+  all.dim = D;
+  for (int32_t k = 0; k < nt; ++k) {
+    for (int a = 0; a < 3; ++a)
+      if (hosts[size_t(k)].size[a] != hosts[0].size[a]) {
+        B->tile_error = "trees differ in their coarsest cell size";
+        return;
+      }
+    all.lmax = std::max(all.lmax, hosts[size_t(k)].lmax);
+  }
+  for (int a = 0; a < 3; ++a) all.size[a] = hosts[0].size[a];
+  all.tile_start.push_back(0);
+  all.halo_start.push_back(0);
+  all.rec_start.push_back(0);
+  std::vector<int32_t> tree_tile(static_cast<size_t>(nt) + 1, 0);
+  const int C = 1 << (D - 1);
+  for (int32_t k = 0; k < nt; ++k) {
+    const TileHost &H = hosts[size_t(k)];
+    const int32_t node_off = int32_t(off[size_t(k)]);
+    const int32_t halo_off = int32_t(all.halo_idx.size());
+    const int32_t rec_off = int32_t(all.rec.size());
+    const uint32_t ext_off = uint32_t(all.ext.size() / size_t(F * W));
+    const int64_t ntile = int64_t(H.tile_start.size()) - 1;
+    tree_tile[size_t(k)] = int32_t(all.tile_start.size()) - 1;
+    for (int64_t t = 0; t < ntile; ++t) {
+      all.tile_start.push_back(H.tile_start[size_t(t) + 1] + node_off);
+      all.halo_start.push_back(H.halo_start[size_t(t) + 1] + halo_off);
+      all.rec_start.push_back(H.rec_start[size_t(t) + 1] + rec_off);
+      all.h1count.push_back(H.h1count[size_t(t)]);
+    }
+    for (int32_t g : H.halo_idx) all.halo_idx.push_back(g + node_off);
+    for (TileRec r : H.rec) {
+      if (r.geo & GEO_IRREGULAR) {
+        const uint32_t id = (uint32_t(r.idx[0]) | (uint32_t(r.idx[1]) << 16)) + ext_off;
+        r.idx[0] = uint16_t(id & 0xffffu);
+        r.idx[1] = uint16_t(id >> 16);
+      }
+      all.rec.push_back(r);
+    }
+    all.rec_slot.insert(all.rec_slot.end(), H.rec_slot.begin(), H.rec_slot.end());
+    const uint32_t scale = 1u << (all.lmax - H.lmax);
+    for (size_t q = 0; q < H.ext.size(); ++q) {
+      uint32_t v = H.ext[q];
+      if (int(q % size_t(W)) >= C) {  // transverse offset words of a face extension
+        v *= scale;
+        if (v > 0xffffu) {
+          B->tile_error = "extension offset does not fit the batch's finest unit";
+          return;
+        }
+      }
+      all.ext.push_back(uint16_t(v));
+    }
+    all.n_irregular += H.n_irregular;
+  }
+  tree_tile[size_t(nt)] = int32_t(all.tile_start.size()) - 1;
+  const int64_t N = off[size_t(nt)];
+  validate_tile_host(all, D, N);
+  upload_tile_host(all, N, &B->st);
+  B->tree_tile.ensure(tree_tile.size());
+  HIP_CHECK(hipMemcpy(B->tree_tile.p, tree_tile.data(), tree_tile.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  B->tiled = true;
 }
 
 void batch_run(casl_batch *B, double *phi, int n_iter, int precision, hipStream_t s, bool allocate_only = false) {
@@ -292,6 +539,7 @@ int casl_batch_create(const casl_flat_stencil *trees, int32_t n_trees, casl_batc
     B->queue.ensure(1);
     HIP_CHECK(hipMemcpy(B->off.p, off.data(), off.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     HIP_CHECK(hipMemcpy(B->order.p, order.data(), order.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    build_batch_tiles(B.get(), trees, off);
     hipDeviceProp_t prop;
     HIP_CHECK(hipGetDeviceProperties(&prop, B->st.device));
     B->cus = prop.multiProcessorCount;
@@ -316,11 +564,26 @@ int casl_batch_info(const casl_batch_t *b, int64_t *total_nodes, int32_t *n_tree
 int casl_batch_set_schedule(casl_batch_t *b, int32_t block_threads, int32_t nodes_per_thread) {
   return guarded([&]() {
     if (!b) throw std::invalid_argument("null batch");
+    if (nodes_per_thread < 0) throw std::invalid_argument("nodes per thread must be >= 0");
     Schedule s;
-    s.block = block_threads > 0 ? block_threads : 256;
-    s.npt = nodes_per_thread > 0 ? nodes_per_thread : 1;
-    (void)pick_kernel<2, double>(s);  // validates the pair
+    s.block = block_threads > 0 ? block_threads : (nodes_per_thread == 0 ? 512 : 256);
+    s.npt = nodes_per_thread;
+    if (s.npt == 0) {
+      if (!b->tiled) throw std::invalid_argument("the tiled batch kernel is not available: " + b->tile_error);
+      if (!(b->st.dim == 2 ? tile_block_ok<2>(s.block) : tile_block_ok<3>(s.block)))
+        throw std::invalid_argument("tiled batch kernel: block must be 512 (or 256 in 2D)");
+    } else {
+      (void)pick_kernel<2, double>(s);  // validates the pair
+    }
     b->sched = s;
+  });
+}
+
+int casl_batch_tile_info(const casl_batch_t *b, int32_t *tiled, int64_t *n_tiles) {
+  return guarded([&]() {
+    if (!b) throw std::invalid_argument("null batch");
+    if (tiled) *tiled = b->tiled ? 1 : 0;
+    if (n_tiles) *n_tiles = b->tiled ? b->st.n_tiles : 0;
   });
 }
 
@@ -384,9 +647,17 @@ int casl_batch_autotune(casl_batch_t *b, const double *phi, int n_iter, int prec
     HIP_CHECK(hipEventCreate(&e1));
     Schedule best = b->sched;
     float best_t = 1e30f;
+    std::vector<Schedule> cands;
+    for (int bt : kBlocks)
+      for (int npt : kNpts) cands.push_back(Schedule{bt, npt});
+    if (b->tiled) {
+      for (int bt : {256, 512})
+        if (b->st.dim == 2 ? tile_block_ok<2>(bt) : tile_block_ok<3>(bt)) cands.push_back(Schedule{bt, 0});
+    }
     try {
-      for (int bt : kBlocks)
-        for (int npt : kNpts) {
+      for (const Schedule &c : cands) {
+        {
+          const int bt = c.block, npt = c.npt;
           b->sched.block = bt;
           b->sched.npt = npt;
           float tsum = 0;
@@ -407,6 +678,7 @@ int casl_batch_autotune(casl_batch_t *b, const double *phi, int n_iter, int prec
             best.npt = npt;
           }
         }
+      }
     } catch (...) {
       (void)hipFree(work);
       (void)hipEventDestroy(e0);

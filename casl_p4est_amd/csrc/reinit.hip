@@ -267,61 +267,55 @@ void validate_tables(const casl_flat_stencil *tab) {
 
 namespace casl {
 
-// Builds and uploads the tiled encoding; on failure the stencil keeps only
-// the grid-wide tables (two-pass kernels) and records why.
-void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
-  st->tiled = false;
-  TileHost H;
-  try {
-    build_tiles(*tab, H);
-  } catch (const std::exception &e) {
-    st->tile_error = e.what();
-    return;
-  }
-  // independent bounds check of every record before anything reaches the GPU
-  {
-    const int D = tab->dim, F = 2 * D, C = 1 << (D - 1), W = (D == 2 ? 3 : 6);
-    const int LMAX = D == 2 ? TileLimits<2>::LMAX : TileLimits<3>::LMAX;
-    const int DMAX = D == 2 ? TileLimits<2>::DMAX : TileLimits<3>::DMAX;
-    const size_t n_ext = H.ext.size() / size_t(F * W);
-    const int64_t nt = int64_t(H.tile_start.size()) - 1;
-    if (H.tile_start.front() != 0 || H.tile_start.back() != tab->n_nodes)
-      throw std::runtime_error("tile ranges do not cover the nodes");
-    for (int64_t t = 0; t < nt; ++t) {
-      const int nT = H.tile_start[t + 1] - H.tile_start[t];
-      const int nH = H.halo_start[t + 1] - H.halo_start[t];
-      const int nH1 = H.h1count[t];
-      const int nR = H.rec_start[t + 1] - H.rec_start[t];
-      if (nT <= 0 || nH1 < 0 || nH1 > nH || nT + nH > LMAX || nT + nH1 > DMAX || nR != nT + nH1)
-        throw std::runtime_error("tile exceeds its LDS limits");
-      for (int k = 0; k < nH; ++k) {
-        const int32_t g = H.halo_idx[size_t(H.halo_start[t]) + k];
-        if (g < 0 || g >= tab->n_nodes) throw std::runtime_error("halo node out of range");
-      }
-      std::vector<char> seen(size_t(nR), 0);
-      for (int j = 0; j < nR; ++j) {
-        const TileRec &rr = H.rec[size_t(H.rec_start[t]) + j];
-        const int slot = H.rec_slot[size_t(H.rec_start[t]) + j];
-        // the first nT records are the tile's own nodes (slots < nT), the rest
-        // its H1 nodes (slots nT .. nT + nH1 - 1), each slot exactly once
-        if ((j < nT) ? (slot >= nT) : (slot < nT || slot >= nT + nH1)) throw std::runtime_error("record slot out of range");
-        if (seen[size_t(slot)]++) throw std::runtime_error("record slot repeated");
-        // pass A of any record reads phi slots < nT + nH; pass B of a tile
-        // node also reads second-derivative slots < nT + nH1 at the same corners
-        const int bound = (j < nT) ? nT + nH1 : nT + nH;
-        if (rr.geo & GEO_IRREGULAR) {
-          const size_t id = size_t(rr.idx[0]) | (size_t(rr.idx[1]) << 16);
-          if (id >= n_ext) throw std::runtime_error("extension id out of range");
-          for (int f = 0; f < F; ++f)
-            for (int c = 0; c < C; ++c)
-              if (H.ext[id * F * W + size_t(f) * W + c] >= bound) throw std::runtime_error("corner slot out of range");
-        } else {
-          for (int f = 0; f < F; ++f)
-            if (rr.idx[f] >= bound) throw std::runtime_error("corner slot out of range");
-        }
+// Independent bounds check of every record of a tiled encoding before
+// anything reaches the GPU: tiles within their LDS limits, halo nodes and
+// extension ids in range, every slot used once, every corner slot readable.
+void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
+  const int F = 2 * D, C = 1 << (D - 1), W = (D == 2 ? 3 : 6);
+  const int LMAX = D == 2 ? TileLimits<2>::LMAX : TileLimits<3>::LMAX;
+  const int DMAX = D == 2 ? TileLimits<2>::DMAX : TileLimits<3>::DMAX;
+  const size_t n_ext = H.ext.size() / size_t(F * W);
+  const int64_t nt = int64_t(H.tile_start.size()) - 1;
+  if (H.tile_start.front() != 0 || H.tile_start.back() != n_nodes)
+    throw std::runtime_error("tile ranges do not cover the nodes");
+  for (int64_t t = 0; t < nt; ++t) {
+    const int nT = H.tile_start[t + 1] - H.tile_start[t];
+    const int nH = H.halo_start[t + 1] - H.halo_start[t];
+    const int nH1 = H.h1count[t];
+    const int nR = H.rec_start[t + 1] - H.rec_start[t];
+    if (nT <= 0 || nH1 < 0 || nH1 > nH || nT + nH > LMAX || nT + nH1 > DMAX || nR != nT + nH1)
+      throw std::runtime_error("tile exceeds its LDS limits");
+    for (int k = 0; k < nH; ++k) {
+      const int32_t g = H.halo_idx[size_t(H.halo_start[t]) + k];
+      if (g < 0 || g >= n_nodes) throw std::runtime_error("halo node out of range");
+    }
+    std::vector<char> seen(size_t(nR), 0);
+    for (int j = 0; j < nR; ++j) {
+      const TileRec &rr = H.rec[size_t(H.rec_start[t]) + j];
+      const int slot = H.rec_slot[size_t(H.rec_start[t]) + j];
+      // the first nT records are the tile's own nodes (slots < nT), the rest
+      // its H1 nodes (slots nT .. nT + nH1 - 1), each slot exactly once
+      if ((j < nT) ? (slot >= nT) : (slot < nT || slot >= nT + nH1)) throw std::runtime_error("record slot out of range");
+      if (seen[size_t(slot)]++) throw std::runtime_error("record slot repeated");
+      // pass A of any record reads phi slots < nT + nH; pass B of a tile
+      // node also reads second-derivative slots < nT + nH1 at the same corners
+      const int bound = (j < nT) ? nT + nH1 : nT + nH;
+      if (rr.geo & GEO_IRREGULAR) {
+        const size_t id = size_t(rr.idx[0]) | (size_t(rr.idx[1]) << 16);
+        if (id >= n_ext) throw std::runtime_error("extension id out of range");
+        for (int f = 0; f < F; ++f)
+          for (int c = 0; c < C; ++c)
+            if (H.ext[id * F * W + size_t(f) * W + c] >= bound) throw std::runtime_error("corner slot out of range");
+      } else {
+        for (int f = 0; f < F; ++f)
+          if (rr.idx[f] >= bound) throw std::runtime_error("corner slot out of range");
       }
     }
   }
+}
+
+// Uploads a validated tiled encoding of n_nodes nodes into st (single geometry).
+void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st) {
   auto up = [](auto &buf, const auto &vec) {
     using T = typename std::decay_t<decltype(vec)>::value_type;
     buf.ensure(vec.size());
@@ -338,7 +332,7 @@ void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
   up(st->t_ext, H.ext.empty() ? std::vector<uint16_t>(8, 0) : H.ext);
   // node -> position of its record among the tile records (tile start + rank)
   {
-    const int64_t N = tab->n_nodes;
+    const int64_t N = n_nodes;
     std::vector<int32_t> pos(size_t(N), -1);
     for (int64_t tt = 0; tt + 1 < int64_t(H.tile_start.size()); ++tt) {
       const int32_t s0 = H.tile_start[tt], nT = H.tile_start[tt + 1] - s0, rb = H.rec_start[tt];
@@ -357,6 +351,21 @@ void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
     st->geom_unit[a] = H.size[a] * std::ldexp(1.0, -H.lmax);
   }
   st->tiled = true;
+}
+
+// Builds and uploads the tiled encoding; on failure the stencil keeps only
+// the grid-wide tables (two-pass kernels) and records why.
+void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
+  st->tiled = false;
+  TileHost H;
+  try {
+    build_tiles(*tab, H);
+  } catch (const std::exception &e) {
+    st->tile_error = e.what();
+    return;
+  }
+  validate_tile_host(H, tab->dim, tab->n_nodes);
+  upload_tile_host(H, tab->n_nodes, st);
 }
 
 void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles) {

@@ -14,6 +14,7 @@
 
 #include "../../include/casl_reinit.h"
 #include "stencil_math.hpp"
+#include "tiles.hpp"
 
 namespace casl {
 void set_error(const std::string &msg);
@@ -338,6 +339,10 @@ namespace casl {
 // Validates and uploads host tables into st (shared by stencils and batches).
 // with_tiles also builds the tiled encoding for the fused stage kernel.
 void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles);
+// Bounds check of a tiled encoding of n_nodes nodes (throws), and its upload
+// into st (tile arrays, the node -> record-position map, one geometry).
+void validate_tile_host(const TileHost &H, int D, int64_t n_nodes);
+void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st);
 
 // Kernel timing (casl_profile_*): prof_begin/prof_end bracket one launch.
 bool prof_on();

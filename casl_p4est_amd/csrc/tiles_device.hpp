@@ -149,16 +149,16 @@ constexpr int TILE_BLOCK = CASL_TILE_BLOCK;
 #ifndef CASL_TILE_WAVES_F64
 #define CASL_TILE_WAVES_F64 3
 #endif
-template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
-__global__ void __launch_bounds__(TILE_BLOCK)
-    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
-    k_tile_stage(TileArgs<R> A) {
-  constexpr int LMAX = TileLimits<D>::LMAX, DMAX = TileLimits<D>::DMAX;
-  constexpr int KT = TileLimits<D>::TMAX / TILE_BLOCK;  // tile nodes per thread, at most
-  static_assert(TileLimits<D>::TMAX % TILE_BLOCK == 0, "tile size must be a multiple of the block");
-  __shared__ R phis[LMAX];
-  __shared__ R dds[SECOND_ORDER ? D * DMAX : 1];
-  const int tile = xcd_tile(int(blockIdx.x), int(gridDim.x));
+// The stage body for one tile, run by a block of BLOCK threads with LDS
+// arrays phis[LMAX] and dds[D * DMAX].  It reads and writes global memory and
+// the LDS arrays only; a caller running several tiles in a row separates them
+// with a barrier (the next tile overwrites the LDS arrays).
+template <int D, class R, bool SECOND_ORDER, bool RK_AVG, int BLOCK>
+__device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__restrict__ phis,
+                                           R *__restrict__ dds) {
+  constexpr int DMAX = TileLimits<D>::DMAX;
+  constexpr int KT = TileLimits<D>::TMAX / BLOCK;  // tile nodes per thread, at most
+  static_assert(TileLimits<D>::TMAX % BLOCK == 0, "tile size must be a multiple of the block");
   const int s = A.tile_start[tile], nT = A.tile_start[tile + 1] - s;
   const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
   const int nH1 = A.h1count[tile];
@@ -169,19 +169,19 @@ __global__ void __launch_bounds__(TILE_BLOCK)
   R tprev[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
-    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    const int i = int(threadIdx.x) + k * BLOCK;
     if (i < nT) {
       trec[k] = A.rec[rb + i];
       tslot[k] = A.rec_slot[rb + i];
       tfl[k] = A.flags[s + i];
     }
   }
-  for (int j = threadIdx.x; j < nT; j += TILE_BLOCK) phis[j] = A.phin[s + j];
-  for (int j = threadIdx.x; j < nH; j += TILE_BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
+  for (int j = threadIdx.x; j < nT; j += BLOCK) phis[j] = A.phin[s + j];
+  for (int j = threadIdx.x; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   if constexpr (RK_AVG) {
 #pragma unroll
     for (int k = 0; k < KT; ++k)
-      if (int(threadIdx.x) + k * TILE_BLOCK < nT) tprev[k] = A.out[s + tslot[k]];
+      if (int(threadIdx.x) + k * BLOCK < nT) tprev[k] = A.out[s + tslot[k]];
   }
   __syncthreads();
   if constexpr (SECOND_ORDER) {
@@ -201,13 +201,13 @@ __global__ void __launch_bounds__(TILE_BLOCK)
     };
 #pragma unroll
     for (int k = 0; k < KT; ++k)
-      if (int(threadIdx.x) + k * TILE_BLOCK < nT) pass_a(trec[k], tslot[k]);
-    for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
+      if (int(threadIdx.x) + k * BLOCK < nT) pass_a(trec[k], tslot[k]);
+    for (int i = nT + int(threadIdx.x); i < nT + nH1; i += BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
     __syncthreads();
   }
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
-    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    const int i = int(threadIdx.x) + k * BLOCK;
     if (i >= nT) continue;
     const TileRec &rec = trec[k];
     const int j = tslot[k];
@@ -231,6 +231,15 @@ __global__ void __launch_bounds__(TILE_BLOCK)
     else
       A.out[n] = r;
   }
+}
+
+template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
+__global__ void __launch_bounds__(TILE_BLOCK)
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
+    k_tile_stage(TileArgs<R> A) {
+  __shared__ R phis[TileLimits<D>::LMAX];
+  __shared__ R dds[SECOND_ORDER ? D * TileLimits<D>::DMAX : 1];
+  tile_stage<D, R, SECOND_ORDER, RK_AVG, TILE_BLOCK>(A, xcd_tile(int(blockIdx.x), int(gridDim.x)), phis, dds);
 }
 
 }  // namespace dev

@@ -149,13 +149,21 @@ int casl_batch_create(const casl_flat_stencil *trees, int32_t n_trees,
 int casl_batch_destroy(casl_batch_t *b);
 int casl_batch_info(const casl_batch_t *b, int64_t *total_nodes,
                     int32_t *n_trees, int32_t *max_tree_nodes);
-/* Launch schedule: threads per block and nodes per thread (0 = defaults). */
+/* Launch schedule.  nodes_per_thread 1 or 2: the kernel that gathers through
+ * the grid-wide tables, block_threads in {64, 128, 256, 512};
+ * nodes_per_thread 0: the kernel that runs each tree's stages tile by tile in
+ * LDS (compact records), block_threads 512 (or 256 in 2D).  block_threads 0
+ * picks the default for the kind. */
 int casl_batch_set_schedule(casl_batch_t *b, int32_t block_threads,
                             int32_t nodes_per_thread);
+/* tiled = 1 when every tree has a tiled encoding (nodes_per_thread 0 usable);
+ * n_tiles = tiles over all trees. */
+int casl_batch_tile_info(const casl_batch_t *b, int32_t *tiled,
+                         int64_t *n_tiles);
 int casl_batch_get_schedule(const casl_batch_t *b, int32_t *block_threads,
                             int32_t *nodes_per_thread);
-/* Times every supported launch shape on a copy of phi (phi is not modified),
- * keeps the fastest as the batch's schedule and reports it. */
+/* Times every supported launch shape (both kernels) on a copy of phi (phi is
+ * not modified), keeps the fastest as the batch's schedule and reports it. */
 int casl_batch_autotune(casl_batch_t *b, const double *phi, int n_iter,
                         int precision, void *stream, int32_t *best_block,
                         int32_t *best_nodes_per_thread, float *best_ms);

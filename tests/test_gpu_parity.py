@@ -187,11 +187,13 @@ def batch_trees(n, seed=17):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_batch_matches_single_tree_path(cuda, precision):
+@pytest.mark.parametrize("schedule", [(256, 1), (512, 0)], ids=["table_kernel", "tile_kernel"])
+def test_batch_matches_single_tree_path(cuda, precision, schedule):
     import torch
-    say(f"[gpu] batch vs single {precision}")
+    say(f"[gpu] batch vs single {precision} schedule {schedule}")
     tabs, phis = batch_trees(24)
     batch = C.TreeBatch(tabs)
+    batch.set_schedule(*schedule)
     phi = torch.from_numpy(np.concatenate(phis)).to(cuda)
     batch.reinitialize_2nd_order(phi, 9, precision=precision)
     out = phi.cpu().numpy()
@@ -227,7 +229,8 @@ def test_batch_graph_replay_and_autotune(cuda):
     bt, npt, ms = batch.autotune(base, 6, "mixed")
     say(f"    autotune: block={bt} nodes/thread={npt} {ms:.3f} ms")
     assert batch.schedule() == (bt, npt)
-    for bt_, npt_ in ((64, 2), (512, 1)):
+    assert batch.tile_info()["tiled"]
+    for bt_, npt_ in ((64, 2), (512, 1), (512, 0), (256, 0)):
         batch.set_schedule(bt_, npt_)
         phi.copy_(base)
         batch.reinitialize_2nd_order(phi, 6, precision="mixed")

@@ -288,7 +288,7 @@ def kernel_profile(fn, torch):
     fn()
     torch.cuda.synchronize()
     out = {}
-    for k in ("k_dd", "k_prep", "k_stage", "k_tile_stage", "k_batch"):
+    for k in ("k_dd", "k_prep", "k_tile_prep", "k_stage", "k_tile_stage", "k_batch"):
         ms, cnt = ctypes.c_double(), ctypes.c_int64()
         L.call("casl_profile_query", k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
         if cnt.value:

@@ -91,7 +91,7 @@ __global__ void __launch_bounds__(BT) k_batch(BatchArgs<D, R> A) {
     for_nodes<BT, NPT>(b, e, [&](int64_t n) { node_dd<D, double>(t, n, A.phi0, A.dd0); });
     __syncthreads();
     for_nodes<BT, NPT>(b, e, [&](int64_t n) {
-      node_prep<D, R>(t, n, n, A.phi0, A.dd0, nullptr, -DBL_MAX, DBL_MAX, A.zdt, true, A.flags, A.sfix);
+      node_prep<D, R>(t, n, A.phi0, A.dd0, nullptr, -DBL_MAX, DBL_MAX, A.zdt, true, A.flags, A.sfix);
     });
     __syncthreads();
     for (int it = 0; it < A.n_iter; ++it) {
@@ -130,9 +130,8 @@ struct BatchTileArgs {
   int n_iter;
 };
 
-// The phi_0-only prep of every batch node (grid-wide, before the persistent
-// kernel): phi_0 and the working copy, pass A on phi_0, flags and arm lengths
-// in tile-record order.
+// phi_0 and the working copy of every batch node (before the tiled prep and
+// the persistent kernel).
 template <class R>
 __global__ void k_batch_copy(const double *__restrict__ phi, double *__restrict__ phi0, R *__restrict__ cur,
                              int64_t N) {
@@ -141,18 +140,6 @@ __global__ void k_batch_copy(const double *__restrict__ phi, double *__restrict_
   const double v = phi[n];
   phi0[n] = v;
   cur[n] = R(v);
-}
-template <int D>
-__global__ void k_batch_dd0(Tables<D> t, const double *__restrict__ phi0, double *__restrict__ dd0) {
-  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (n < t.N) node_dd<D, double>(t, n, phi0, dd0);
-}
-template <int D, class R>
-__global__ void k_batch_prep(Tables<D> t, const double *__restrict__ phi0, const double *__restrict__ dd0,
-                             const int32_t *__restrict__ pos, double zdt, uint16_t *__restrict__ flags,
-                             R *__restrict__ sfix) {
-  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (n < t.N) node_prep<D, R>(t, n, pos[n], phi0, dd0, nullptr, -DBL_MAX, DBL_MAX, zdt, true, flags, sfix);
 }
 
 // Persistent kernel over trees whose pseudo-time stages run tile by tile in
@@ -310,22 +297,7 @@ void launch_batch(casl_batch *B, double *phi, int n_iter, hipStream_t s, bool al
     if (!B->tiled) throw std::invalid_argument("the tiled batch kernel is not available: " + B->tile_error);
     BatchTileArgs<R> P{};
     TileArgs<R> &T = P.T;
-    T.tile_start = st.t_tile_start.p;
-    T.halo_start = st.t_halo_start.p;
-    T.h1count = st.t_h1count.p;
-    T.halo = st.t_halo.p;
-    T.rec_start = st.t_rec_start.p;
-    T.rec = reinterpret_cast<const TileRec *>(st.t_rec.p);
-    T.rec_slot = st.t_rec_slot.p;
-    T.ext = st.t_ext.p;
-    T.geom.pow2 = 1;
-    for (int a = 0; a < 3; ++a) {
-      T.geom.base[a] = R(st.geom_base[a]);
-      T.geom.unit[a] = R(st.geom_unit[a]);
-      T.geom.inv_base[a] = R(1.0 / st.geom_base[a]);
-      int ex = 0;
-      if (std::frexp(st.geom_base[a], &ex) != 0.5) T.geom.pow2 = 0;
-    }
+    T = tile_args<R>(&st);
     T.flags = st.flags.p;
     T.sfix = A.sfix;
     T.phi0 = st.phi0.p;
@@ -341,9 +313,11 @@ void launch_batch(casl_batch *B, double *phi, int n_iter, hipStream_t s, bool al
     P.n_iter = n_iter;
     const unsigned g = unsigned((N + 255) / 256);
     hipLaunchKernelGGL((k_batch_copy<R>), dim3(g), dim3(256), 0, s, phi, st.phi0.p, A.cur, N);
-    hipLaunchKernelGGL((k_batch_dd0<D>), dim3(g), dim3(256), 0, s, A.t, st.phi0.p, st.dd0.p);
-    hipLaunchKernelGGL((k_batch_prep<D, R>), dim3(g), dim3(256), 0, s, A.t, st.phi0.p, st.dd0.p, st.t_pos.p,
-                       st.zdt, st.flags.p, A.sfix);
+    // flags and s_fix of every tree in tile-record order (FP64 tiled prep)
+    TileArgs<double> tp = tile_args<double>(&st);
+    tp.phin = st.phi0.p;
+    hipLaunchKernelGGL((k_tile_prep<D, R, true>), dim3(unsigned(st.n_tiles)), dim3(TILE_BLOCK), 0, s, tp,
+                       (const uint8_t *)nullptr, -DBL_MAX, DBL_MAX, st.zdt, st.flags.p, A.sfix);
     HIP_CHECK(hipGetLastError());
     TileKernelFn<D, R> fn = pick_tile_kernel<D, R>(B->sched.block);
     int occ = 0;

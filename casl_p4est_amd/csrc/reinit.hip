@@ -48,11 +48,10 @@ template <int D, class S>
 __global__ void __launch_bounds__(BLOCK)
     k_prep(Tables<D> t, const double *__restrict__ phi0, const double *__restrict__ dd0,
            const uint8_t *__restrict__ mask, double lim_low, double lim_high, double zdt, int second_order,
-           const int32_t *__restrict__ pos, uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
+           uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
-  const int64_t m = pos ? int64_t(pos[n]) : n;
-  node_prep<D, S>(t, n, m, phi0, dd0, mask, lim_low, lim_high, zdt, second_order != 0, flags, s_fix);
+  node_prep<D, S>(t, n, phi0, dd0, mask, lim_low, lim_high, zdt, second_order != 0, flags, s_fix);
 }
 
 // Pass B of one pseudo-time stage.  out[n] = result, or for the second RK2
@@ -105,6 +104,40 @@ void require_device() {
   if (e != hipSuccess || count == 0) throw hip_error("no HIP device available");
 }
 
+}  // namespace
+
+namespace casl {
+// The tile arrays and geometry of a stencil's tiled encoding, in arithmetic R
+// (per-call fields -- flags, s_fix, phi_0, phi in/out -- are the caller's).
+template <class R>
+dev::TileArgs<R> tile_args(const casl_stencil *st) {
+  dev::TileArgs<R> ta{};
+  ta.tile_start = st->t_tile_start.p;
+  ta.halo_start = st->t_halo_start.p;
+  ta.h1count = st->t_h1count.p;
+  ta.halo = st->t_halo.p;
+  ta.rec_start = st->t_rec_start.p;
+  ta.rec = reinterpret_cast<const TileRec *>(st->t_rec.p);
+  ta.rec_slot = st->t_rec_slot.p;
+  ta.ext = st->t_ext.p;
+  ta.geom.pow2 = 1;
+  for (int a = 0; a < 3; ++a) {
+    ta.geom.base[a] = R(st->geom_base[a]);
+    ta.geom.unit[a] = R(st->geom_unit[a]);
+    int e = 0;
+    const double m = std::frexp(st->geom_base[a], &e);
+    ta.geom.inv_base[a] = R(1.0 / st->geom_base[a]);
+    if (m != 0.5) ta.geom.pow2 = 0;
+  }
+  ta.N = st->N;
+  return ta;
+}
+template dev::TileArgs<float> tile_args<float>(const casl_stencil *);
+template dev::TileArgs<double> tile_args<double>(const casl_stencil *);
+}  // namespace casl
+
+namespace {
+
 // reinitialize_within_range_of_phi_0 driver, templated on dimension and the
 // stage arithmetic type R (double: FP64, float: mixed).
 template <int D, class R>
@@ -118,11 +151,12 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   constexpr int F = 2 * D;
   const unsigned grid = grid_for(N);
 
+  const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
   st->phi0.ensure(N);
   st->flags.ensure(N);
-  if (so) st->dd0.ensure(size_t(D) * N);
   HIP_CHECK(hipMemcpyAsync(st->phi0.p, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-  if (so) {
+  if (so && !tiled) {
+    st->dd0.ensure(size_t(D) * N);
     prof_begin("k_dd", s);
     hipLaunchKernelGGL((k_dd<D, double>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p);
     prof_end("k_dd", s);
@@ -149,13 +183,25 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
     hipLaunchKernelGGL(k_d2f, dim3(grid), dim3(BLOCK), 0, s, phi, st->phif.p, N);
     HIP_CHECK(hipGetLastError());
   }
-  // the tiled stage kernel reads flags and s_fix in its record order
-  const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
-  const int32_t *pos = tiled ? st->t_pos.p : nullptr;
-  prof_begin("k_prep", s);
-  hipLaunchKernelGGL((k_prep<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p, mask, lim_low,
-                     lim_high, st->zdt, int(so), pos, st->flags.p, sfix);
-  prof_end("k_prep", s);
+  if (tiled) {
+    // flags and s_fix in tile-record order, the order the tiled stages read them
+    TileArgs<double> tp = tile_args<double>(st);
+    tp.phin = st->phi0.p;
+    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK);
+    prof_begin("k_tile_prep", s);
+    if (so)
+      hipLaunchKernelGGL((k_tile_prep<D, R, true>), g, b, 0, s, tp, mask, lim_low, lim_high, st->zdt, st->flags.p,
+                         sfix);
+    else
+      hipLaunchKernelGGL((k_tile_prep<D, R, false>), g, b, 0, s, tp, mask, lim_low, lim_high, st->zdt, st->flags.p,
+                         sfix);
+    prof_end("k_tile_prep", s);
+  } else {
+    prof_begin("k_prep", s);
+    hipLaunchKernelGGL((k_prep<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p, mask, lim_low,
+                       lim_high, st->zdt, int(so), st->flags.p, sfix);
+    prof_end("k_prep", s);
+  }
   HIP_CHECK(hipGetLastError());
 
   auto dd_pass = [&](const R *in) {
@@ -187,27 +233,10 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   };
   TileArgs<R> ta{};
   if (tiled) {
-    ta.tile_start = st->t_tile_start.p;
-    ta.halo_start = st->t_halo_start.p;
-    ta.h1count = st->t_h1count.p;
-    ta.halo = st->t_halo.p;
-    ta.rec_start = st->t_rec_start.p;
-    ta.rec = reinterpret_cast<const TileRec *>(st->t_rec.p);
-    ta.rec_slot = st->t_rec_slot.p;
-    ta.ext = st->t_ext.p;
-    ta.geom.pow2 = 1;
-    for (int a = 0; a < 3; ++a) {
-      ta.geom.base[a] = R(st->geom_base[a]);
-      ta.geom.unit[a] = R(st->geom_unit[a]);
-      int e = 0;
-      const double m = std::frexp(st->geom_base[a], &e);
-      ta.geom.inv_base[a] = R(1.0 / st->geom_base[a]);
-      if (m != 0.5) ta.geom.pow2 = 0;
-    }
+    ta = tile_args<R>(st);
     ta.flags = st->flags.p;
     ta.sfix = sfix;
     ta.phi0 = st->phi0.p;
-    ta.N = N;
   }
   auto tile_stage = [&](const R *in, R *out, bool avg) {
     ta.phin = in;
@@ -316,6 +345,7 @@ void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
 
 // Uploads a validated tiled encoding of n_nodes nodes into st (single geometry).
 void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st) {
+  if (H.tile_start.empty() || H.tile_start.back() != n_nodes) throw std::runtime_error("tile ranges do not cover the nodes");
   auto up = [](auto &buf, const auto &vec) {
     using T = typename std::decay_t<decltype(vec)>::value_type;
     buf.ensure(vec.size());
@@ -330,18 +360,6 @@ void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st) {
   st->t_rec.ensure(H.rec.size());
   HIP_CHECK(hipMemcpy(st->t_rec.p, H.rec.data(), H.rec.size() * sizeof(TileRec), hipMemcpyHostToDevice));
   up(st->t_ext, H.ext.empty() ? std::vector<uint16_t>(8, 0) : H.ext);
-  // node -> position of its record among the tile records (tile start + rank)
-  {
-    const int64_t N = n_nodes;
-    std::vector<int32_t> pos(size_t(N), -1);
-    for (int64_t tt = 0; tt + 1 < int64_t(H.tile_start.size()); ++tt) {
-      const int32_t s0 = H.tile_start[tt], nT = H.tile_start[tt + 1] - s0, rb = H.rec_start[tt];
-      for (int32_t i = 0; i < nT; ++i) pos[size_t(s0 + H.rec_slot[size_t(rb + i)])] = s0 + i;
-    }
-    for (int64_t n = 0; n < N; ++n)
-      if (pos[size_t(n)] < 0) throw std::runtime_error("tile records do not cover every node once");
-    up(st->t_pos, pos);
-  }
   st->n_tiles = int64_t(H.tile_start.size()) - 1;
   st->n_halo = int64_t(H.halo_idx.size());
   st->n_records = int64_t(H.rec.size());

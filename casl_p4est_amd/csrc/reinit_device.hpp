@@ -113,30 +113,31 @@ __device__ __forceinline__ void node_dd(const Tables<D> &t, int64_t n, const R *
 
 // The phi_0-only part of reinitialize_one_iteration (src/my_p4est_level_set.cpp:82-211):
 // mode, sign of phi_0 and the subcell arm lengths of crossing arms,
-// s_fix[f][m] = MAX(interface_Location*(..), zero_distance_threshold).
-// The reference recomputes these identically at every stage.  m is where the
-// node's entries go: n itself, or its position in the tiled record order.
-template <int D, class S>
-__device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, int64_t m, const double *__restrict__ phi0,
-                                          const double *__restrict__ dd0, const uint8_t *__restrict__ mask,
-                                          double lim_low, double lim_high, double zdt, bool second_order,
-                                          uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
-  const double p = phi0[n];
-  if (mask && mask[n] == 0) {
-    flags[m] = MODE_MASKED;
-    return;
-  }
-  if (fabs(p) < zdt) {
-    flags[m] = MODE_ANCHOR;
-    return;
-  }
-  if (!(lim_low < p && p < lim_high)) {
-    flags[m] = MODE_COPY0;
-    return;
-  }
+// s_fix[f] = MAX(interface_Location*(..), zero_distance_threshold).
+// The reference recomputes these identically at every stage.  Returns the
+// node's flags; writes s_fix of its crossing arms at sfix_m[f * sfix_stride].
+//   t, j      stencil accessor and the node's index in it and in phi0 / dd0
+//   masked    the node is outside the mask
+//   dd0       pass A of phi_0, component a at dd0[a * dd_stride + index]
+//   OWN_DD    the node's pass-A values are dd0[a * dd_stride + j]
+template <int D, class S, class TB, bool OWN_DD = false>
+__device__ __forceinline__ uint16_t prep_value(const TB &t, int64_t j, bool masked, const double *__restrict__ phi0,
+                                               const double *__restrict__ dd0, int64_t dd_stride, double lim_low,
+                                               double lim_high, double zdt, bool second_order,
+                                               S *__restrict__ sfix_m, int64_t sfix_stride) {
+  const double p = phi0[j];
+  if (masked) return MODE_MASKED;
+  if (fabs(p) < zdt) return MODE_ANCHOR;
+  if (!(lim_low < p && p < lim_high)) return MODE_COPY0;
   constexpr int F = 2 * D;
   double p0_000, p0_f[F], dloc[D];
-  ngbd_quadratic<D, double>(t, n, phi0, p0_000, p0_f, dloc);
+  if constexpr (OWN_DD) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) dloc[a] = dd0[int64_t(a) * dd_stride + j];
+    ngbd_quadratic_given_dd<D, double>(t, j, phi0, dloc, p0_000, p0_f);
+  } else {
+    ngbd_quadratic<D, double>(t, j, phi0, p0_000, p0_f, dloc);
+  }
   uint16_t fl = MODE_UPDATE | (p0_000 > 0.0 ? 4 : 0);
   unsigned cross = 0;
 #pragma unroll
@@ -147,10 +148,10 @@ __device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, int64_t
     for (int f = 0; f < F; ++f) {
       if (!(cross & (1u << f))) continue;
       const int a = f >> 1;
-      double s = double(t.d(f, n));
+      double s = double(t.d(f, j));
       if (second_order) {
-        const double dd_000 = dd0[int64_t(a) * t.N + n];
-        const double dd_f = face_linear<D, double>(t, f, n, dd0 + int64_t(a) * t.N);
+        const double dd_000 = dd0[int64_t(a) * dd_stride + j];
+        const double dd_f = face_linear<D, double>(t, f, j, dd0 + int64_t(a) * dd_stride);
         if ((f & 1) == 0)
           s = -interface_location_2nd(-s, 0.0, p0_f[f], p0_000, dd_f, dd_000);
         else
@@ -161,11 +162,21 @@ __device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, int64_t
         else
           s = interface_location(0.0, s, p0_000, p0_f[f]);
       }
-      s_fix[int64_t(f) * t.N + m] = S(max2(s, zdt));
+      sfix_m[int64_t(f) * sfix_stride] = S(max2(s, zdt));
     }
     fl |= uint16_t(cross << 3);
   }
-  flags[m] = fl;
+  return fl;
+}
+
+// prep_value on the grid-wide tables, flags and s_fix in node order
+template <int D, class S>
+__device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, const double *__restrict__ phi0,
+                                          const double *__restrict__ dd0, const uint8_t *__restrict__ mask,
+                                          double lim_low, double lim_high, double zdt, bool second_order,
+                                          uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
+  flags[n] = prep_value<D, S>(t, n, mask && mask[n] == 0, phi0, dd0, t.N, lim_low, lim_high, zdt, second_order,
+                              s_fix + n, t.N);
 }
 
 // Pass B at one node: the stage result of reinitialize_one_iteration
@@ -319,7 +330,7 @@ struct casl_stencil {
   int64_t n_tiles = 0, n_halo = 0, n_records = 0, n_irregular = 0;
   double geom_base[3] = {0, 0, 0}, geom_unit[3] = {0, 0, 0};
   std::string tile_error;
-  casl::DevBuf<int32_t> t_tile_start, t_halo_start, t_h1count, t_halo, t_rec_start, t_pos;
+  casl::DevBuf<int32_t> t_tile_start, t_halo_start, t_h1count, t_halo, t_rec_start;
   casl::DevBuf<uint16_t> t_rec_slot;
   casl::DevBuf<uint4> t_rec;
   casl::DevBuf<uint16_t> t_ext;
@@ -340,7 +351,7 @@ namespace casl {
 // with_tiles also builds the tiled encoding for the fused stage kernel.
 void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles);
 // Bounds check of a tiled encoding of n_nodes nodes (throws), and its upload
-// into st (tile arrays, the node -> record-position map, one geometry).
+// into st (tile arrays, one geometry).
 void validate_tile_host(const TileHost &H, int D, int64_t n_nodes);
 void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st);
 

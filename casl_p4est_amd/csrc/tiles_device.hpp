@@ -233,6 +233,72 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   }
 }
 
+// The phi_0-only prep of one tile in FP64: phi_0 of T u H1 u H2 into LDS,
+// pass A of phi_0 for T u H1 (second order), then flags and s_fix of the
+// tile's nodes, written in record order (tile start + record rank), the
+// order the stage body reads them in.  A.phin is phi_0.
+template <int D, class S, bool SECOND_ORDER, int BLOCK>
+__device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, double *__restrict__ phis,
+                                          double *__restrict__ dds, const uint8_t *__restrict__ mask,
+                                          double lim_low, double lim_high, double zdt,
+                                          uint16_t *__restrict__ flags, S *__restrict__ sfix) {
+  constexpr int DMAX = TileLimits<D>::DMAX;
+  const int s = A.tile_start[tile], nT = A.tile_start[tile + 1] - s;
+  const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
+  const int nH1 = A.h1count[tile];
+  const int rb = A.rec_start[tile];
+  for (int j = threadIdx.x; j < nT; j += BLOCK) phis[j] = A.phin[s + j];
+  for (int j = threadIdx.x; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
+  __syncthreads();
+  if constexpr (SECOND_ORDER) {
+    for (int i = int(threadIdx.x); i < nT + nH1; i += BLOCK) {
+      const TileRec rec = A.rec[rb + i];
+      const int j = A.rec_slot[rb + i];
+      double f000, fnb[2 * D], dd[D];
+      if (!(rec.geo & GEO_IRREGULAR)) {
+        RegStencil<D, double> st;
+        st.load(rec, A.geom);
+        laplace<D, double>(st, j, phis, f000, fnb, dd);
+      } else {
+        IrrStencil<D, double> st;
+        st.load(rec, A.ext, A.geom);
+        laplace<D, double>(st, j, phis, f000, fnb, dd);
+      }
+#pragma unroll
+      for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
+    }
+    __syncthreads();
+  }
+  for (int i = int(threadIdx.x); i < nT; i += BLOCK) {
+    const TileRec rec = A.rec[rb + i];
+    const int j = A.rec_slot[rb + i];
+    const bool masked = mask && mask[s + j] == 0;
+    uint16_t fl;
+    if (!(rec.geo & GEO_IRREGULAR)) {
+      RegStencil<D, double> st;
+      st.load(rec, A.geom);
+      fl = prep_value<D, S, RegStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
+                                                                 zdt, SECOND_ORDER, sfix + (s + i), A.N);
+    } else {
+      IrrStencil<D, double> st;
+      st.load(rec, A.ext, A.geom);
+      fl = prep_value<D, S, IrrStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
+                                                                 zdt, SECOND_ORDER, sfix + (s + i), A.N);
+    }
+    flags[s + i] = fl;
+  }
+}
+
+template <int D, class S, bool SECOND_ORDER>
+__global__ void __launch_bounds__(TILE_BLOCK) k_tile_prep(TileArgs<double> A, const uint8_t *__restrict__ mask,
+                                                          double lim_low, double lim_high, double zdt,
+                                                          uint16_t *__restrict__ flags, S *__restrict__ sfix) {
+  __shared__ double phis[TileLimits<D>::LMAX];
+  __shared__ double dds[SECOND_ORDER ? D * TileLimits<D>::DMAX : 1];
+  tile_prep<D, S, SECOND_ORDER, TILE_BLOCK>(A, xcd_tile(int(blockIdx.x), int(gridDim.x)), phis, dds, mask, lim_low,
+                                            lim_high, zdt, flags, sfix);
+}
+
 template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
 __global__ void __launch_bounds__(TILE_BLOCK)
     __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
@@ -243,4 +309,10 @@ __global__ void __launch_bounds__(TILE_BLOCK)
 }
 
 }  // namespace dev
+
+// The tile arrays and geometry of a stencil's tiled encoding in arithmetic R;
+// the per-call fields (flags, s_fix, phi_0, phi in and out) are left empty.
+template <class R>
+dev::TileArgs<R> tile_args(const casl_stencil *st);
+
 }  // namespace casl

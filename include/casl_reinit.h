@@ -133,7 +133,7 @@ int casl_band_linf(casl_stencil_t *st, const double *a, const double *b,
  * When enabled, every kernel launch of the library is bracketed by HIP
  * events recorded on its launch stream.  casl_profile_query returns the
  * summed event time (ms) and the launch count of one kernel ("k_dd",
- * "k_prep", "k_stage", "k_batch") since the last reset; it synchronizes the
+ * "k_prep", "k_tile_prep", "k_stage", "k_tile_stage", "k_batch", "k_gemm") since the last reset; it synchronizes the
  * recorded events.  Off by default (no events, no overhead). */
 int casl_profile_enable(int enable);
 int casl_profile_reset(void);

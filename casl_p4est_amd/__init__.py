@@ -11,11 +11,13 @@ from . import levelset
 from ._lib import CaslError
 from .batch import TreeBatch
 from .forest import FlatStencil, Forest, NodeNeighbors, flat_stencil_from_forest
-from .level_set import DBL_MAX, DeviceStencil, LevelSet, my_p4est_level_set_t
+from .level_set import (DBL_MAX, DeviceStencil, LevelSet, compute_mean_curvature, compute_normals,
+                        my_p4est_level_set_t)
 
 my_p4est_node_neighbors_t = NodeNeighbors
 
 __all__ = [
     "CaslError", "DBL_MAX", "DeviceStencil", "FlatStencil", "Forest", "LevelSet", "NodeNeighbors", "TreeBatch",
+    "compute_mean_curvature", "compute_normals",
     "flat_stencil_from_forest", "levelset", "my_p4est_level_set_t", "my_p4est_node_neighbors_t",
 ]

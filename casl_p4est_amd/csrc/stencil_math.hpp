@@ -325,5 +325,84 @@ __device__ __forceinline__ void ngbd_quadratic_given_dd(const TB &t, int64_t n, 
   quadratic_corrections<D, R>(t, n, fnb, dd);
 }
 
+// ---------------------------------------------------- first derivatives
+// central_derivative, my_p4est_quad_neighbor_nodes_of_node.h:200-206
+template <class R>
+__host__ __device__ __forceinline__ R central_derivative(R fp, R f0, R fm, R dp, R dm) {
+  return ((fp - f0) * dm / dp + (f0 - fm) * dp / dm) / (dp + dm);
+}
+
+// dd_correction_weight_to_naive_first_derivative, my_p4est_quad_neighbor_nodes_of_node.h:186-192
+template <class R>
+__host__ __device__ __forceinline__ R dd_correction_weight(R d_m, R d_p, R d_m_m, R d_m_p, R d_p_m, R d_p_p) {
+  return R(0.5) * (d_p_m * d_p_p * d_m / d_p - d_m_m * d_m_p * d_p / d_m) / (d_m + d_p);
+}
+
+// 1 / (the largest arm or transverse distance), the list construct_neighbors
+// uses (src/my_p4est_node_neighbors.cpp:1168-1186; in 3D its list pairs
+// d_00m_0m with d_00p_0p and leaves d_00m_0p out).
+template <int D, class TB>
+__device__ __forceinline__ double inverse_d_max(const TB &t, int64_t n) {
+  double m = max2(double(t.d(0, n)), double(t.d(1, n)));
+  m = max2(max2(m, double(t.d(2, n))), double(t.d(3, n)));
+  if constexpr (D == 3) m = max2(max2(m, double(t.d(4, n))), double(t.d(5, n)));
+  // face f, transverse k (m0, p0[, 0m, 0p])
+#pragma unroll
+  for (int f = 0; f < 4; ++f) m = max2(max2(m, double(t.dt(f, 0, n))), double(t.dt(f, 1, n)));
+  if constexpr (D == 3) {
+#pragma unroll
+    for (int f = 0; f < 4; ++f) m = max2(max2(m, double(t.dt(f, 2, n))), double(t.dt(f, 3, n)));
+    m = max2(max2(m, double(t.dt(4, 0, n))), double(t.dt(4, 1, n)));  // d_00m_m0, d_00m_p0
+    m = max2(max2(m, double(t.dt(4, 2, n))), double(t.dt(5, 3, n)));  // d_00m_0m, d_00p_0p
+    m = max2(max2(m, double(t.dt(5, 0, n))), double(t.dt(5, 1, n)));  // d_00p_m0, d_00p_p0
+    m = max2(max2(m, double(t.dt(5, 2, n))), double(t.dt(5, 3, n)));  // d_00p_0m, d_00p_0p
+  }
+  return 1.0 / m;
+}
+
+// Central first derivatives of f at one node, dx_central / dy_central /
+// dz_central (my_p4est_quad_neighbor_nodes_of_node.cpp:638-700 and the y, z
+// counterparts): the naive central derivative on the linearly interpolated
+// face values, minus f_tt * weight for each transverse axis t whose weight
+// (scaled by inverse_d_max) is not zero -- f_tt from laplace, computed only
+// when some correction is needed.  FP64.
+template <int D, class G, class TB>
+__device__ __forceinline__ void gradient_central(const TB &t, int64_t n, const G *__restrict__ f, double *grad) {
+  const double f000 = double(f[n]);
+  double fnb[2 * D];
+#pragma unroll
+  for (int q = 0; q < 2 * D; ++q) fnb[q] = face_linear<D, double>(t, q, n, f);
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+    grad[a] = central_derivative(fnb[2 * a + 1], f000, fnb[2 * a], double(t.d(2 * a + 1, n)), double(t.d(2 * a, n)));
+  const double idm = inverse_d_max<D>(t, n);
+  double w[D][2];
+  bool need[D][2];
+  bool any = false;
+#pragma unroll
+  for (int a = 0; a < D; ++a)
+#pragma unroll
+    for (int k = 0; k < D - 1; ++k) {
+      const int fm = 2 * a, fp = 2 * a + 1;
+      w[a][k] = dd_correction_weight(double(t.d(fm, n)), double(t.d(fp, n)), double(t.dt(fm, 2 * k, n)),
+                                     double(t.dt(fm, 2 * k + 1, n)), double(t.dt(fp, 2 * k, n)),
+                                     double(t.dt(fp, 2 * k + 1, n)));
+      need[a][k] = !check_if_zero(w[a][k] * idm);
+      any = any || need[a][k];
+    }
+  if (any) {
+    double f0, fn2[2 * D], dd[D];
+    laplace<D, double>(t, n, f, f0, fn2, dd);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+      for (int k = 0; k < D - 1; ++k) {
+        // transverse axes of axis a, in order: x -> (y, z), y -> (x, z), z -> (x, y)
+        const int ta = (k == 0) ? (a == 0 ? 1 : 0) : (a == 2 ? 1 : 2);
+        if (need[a][k]) grad[a] -= dd[ta] * w[a][k];
+      }
+  }
+}
+
 }  // namespace dev
 }  // namespace casl

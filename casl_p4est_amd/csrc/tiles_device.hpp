@@ -164,16 +164,14 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   const int nH1 = A.h1count[tile];
   const int rb = A.rec_start[tile];
   TileRec trec[KT];
-  int tslot[KT];
-  uint16_t tfl[KT];
+  uint32_t tsf[KT];  // LDS slot (low 16 bits) and flags (high 16 bits)
   R tprev[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * BLOCK;
     if (i < nT) {
       trec[k] = A.rec[rb + i];
-      tslot[k] = A.rec_slot[rb + i];
-      tfl[k] = A.flags[s + i];
+      tsf[k] = uint32_t(A.rec_slot[rb + i]) | (uint32_t(A.flags[s + i]) << 16);
     }
   }
   for (int j = threadIdx.x; j < nT; j += BLOCK) phis[j] = A.phin[s + j];
@@ -181,7 +179,7 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   if constexpr (RK_AVG) {
 #pragma unroll
     for (int k = 0; k < KT; ++k)
-      if (int(threadIdx.x) + k * BLOCK < nT) tprev[k] = A.out[s + tslot[k]];
+      if (int(threadIdx.x) + k * BLOCK < nT) tprev[k] = A.out[s + int(tsf[k] & 0xffffu)];
   }
   __syncthreads();
   if constexpr (SECOND_ORDER) {
@@ -201,7 +199,7 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
     };
 #pragma unroll
     for (int k = 0; k < KT; ++k)
-      if (int(threadIdx.x) + k * BLOCK < nT) pass_a(trec[k], tslot[k]);
+      if (int(threadIdx.x) + k * BLOCK < nT) pass_a(trec[k], int(tsf[k] & 0xffffu));
     for (int i = nT + int(threadIdx.x); i < nT + nH1; i += BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
     __syncthreads();
   }
@@ -210,9 +208,9 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
     const int i = int(threadIdx.x) + k * BLOCK;
     if (i >= nT) continue;
     const TileRec &rec = trec[k];
-    const int j = tslot[k];
+    const int j = int(tsf[k] & 0xffffu);
     const int64_t n = int64_t(s) + j;
-    const uint16_t fl = tfl[k];
+    const uint16_t fl = uint16_t(tsf[k] >> 16);
     const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
     R r;
     if (!(rec.geo & GEO_IRREGULAR)) {

@@ -205,3 +205,7 @@ class NodeNeighbors:
     def second_derivatives_central(self, f, fdd=None):
         """fdd[a][n] = d^2f/dx_a^2 at node n, on the GPU (torch float64 tensors)."""
         return self.device().second_derivatives(f, fdd)
+
+    def first_derivatives_central(self, f, fd=None):
+        """fd[a][n] = df/dx_a at node n, on the GPU (torch float64 tensors)."""
+        return self.device().first_derivatives(f, fd)

@@ -109,6 +109,39 @@ class DeviceStencil:
                    int(n_iter), float(limit_high), float(limit_low), int(precision), st)
         return phi
 
+    def _field_out(self, f, out, comps):
+        torch = _torch()
+        self._check_field(f, "f")
+        if f.numel() != comps[0] * self.n_nodes:
+            raise ValueError("input has the wrong number of values")
+        if out is None:
+            out = torch.empty((comps[1], self.n_nodes), dtype=torch.float64, device=f.device)
+        self._check_field(out, "out")
+        if out.numel() != comps[1] * self.n_nodes:
+            raise ValueError("output has the wrong number of values")
+        return out
+
+    def first_derivatives(self, f, fd=None):
+        """first_derivatives_central: fd[a][n] = d f / dx_a (FP64)."""
+        fd = self._field_out(f, fd, (1, self.dim))
+        L.call("casl_first_derivatives", self._h, ctypes.c_void_p(f.data_ptr()), ctypes.c_void_p(fd.data_ptr()),
+               _stream_ptr(_torch()))
+        return fd
+
+    def normals(self, phi, out=None):
+        """compute_normals: grad(phi) / |grad(phi)|, 0 where |grad(phi)| < EPS."""
+        out = self._field_out(phi, out, (1, self.dim))
+        L.call("casl_normals", self._h, ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+               _stream_ptr(_torch()))
+        return out
+
+    def mean_curvature(self, normals, out=None):
+        """compute_mean_curvature(normals): div(normals) by central derivatives."""
+        out = self._field_out(normals, out, (self.dim, 1))
+        L.call("casl_mean_curvature", self._h, ctypes.c_void_p(normals.data_ptr()), ctypes.c_void_p(out.data_ptr()),
+               _stream_ptr(_torch()))
+        return out.view(-1)
+
     def band_linf(self, a, b, ref, band):
         torch = _torch()
         for t, n in ((a, "a"), (b, "b"), (ref, "ref")):
@@ -216,6 +249,16 @@ class LevelSet:
     def reinitialize_1st_order_time_2nd_order_space_above_threshold(self, phi, threshold, number_of_iterations=50,
                                                                     upper_limit_phi_0=DBL_MAX):
         return self.reinitialize_within_range_of_phi_0(phi, 2, 1, upper_limit_phi_0, threshold, number_of_iterations)
+
+
+def compute_normals(ngbd, phi, normals=None):
+    """compute_normals(neighbors, phi, normals) (src/my_p4est_utils.cpp:1720-1740)."""
+    return ngbd.device().normals(phi, normals)
+
+
+def compute_mean_curvature(ngbd, normals, kappa=None):
+    """compute_mean_curvature(neighbors, normals, kappa) (src/my_p4est_utils.cpp:1648-1690)."""
+    return ngbd.device().mean_curvature(normals, kappa)
 
 
 # the reference's class names

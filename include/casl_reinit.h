@@ -109,6 +109,23 @@ int casl_stencil_tile_info(const casl_stencil_t *st, int32_t *tiled,
 int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd,
                             void *stream);
 
+/* first_derivatives_central: fd[a*N + n] = d f / dx_a at node n -- the central
+ * derivative on linearly interpolated face values, corrected by the second
+ * derivatives where a hanging arm needs it (src/my_p4est_node_neighbors.h:568-600,
+ * src/my_p4est_quad_neighbor_nodes_of_node.cpp:638-760).  FP64 device arrays. */
+int casl_first_derivatives(casl_stencil_t *st, const double *f, double *fd,
+                           void *stream);
+/* compute_normals (src/my_p4est_utils.cpp:1720-1740): normals[a*N + n] =
+ * first derivatives / their norm, 0 where the norm is below EPS. */
+int casl_normals(casl_stencil_t *st, const double *phi, double *normals,
+                 void *stream);
+/* compute_mean_curvature(neighbors, normals, kappa) (src/my_p4est_utils.cpp:
+ * 1648-1690; qnnn::get_curvature(normals), src/my_p4est_quad_neighbor_nodes_of_node.h:
+ * 2167-2171): kappa[n] = sum over a of the central derivative of normals[a]
+ * along axis a. */
+int casl_mean_curvature(casl_stencil_t *st, const double *normals, double *kappa,
+                        void *stream);
+
 /* reinitialize_within_range_of_phi_0(phi, order_space, order_pseudotime,
  * phi_0_limit_high, phi_0_limit_low, number_of_iterations): phi is an FP64
  * device array of n_nodes values, updated in place.  Orders outside {1,2}

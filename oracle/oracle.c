@@ -25,6 +25,15 @@
  *   second_derivatives_central (1 rank)  src/my_p4est_node_neighbors.cpp:1422-1641
  *   reinitialize_one_iteration           src/my_p4est_level_set.cpp:59-288
  *   reinitialize_within_range_of_phi_0   src/my_p4est_level_set.cpp:290-557
+ *   central_derivative                   src/my_p4est_quad_neighbor_nodes_of_node.h:200-206
+ *   dd_correction_weight_to_naive_first_derivative
+ *                                        src/my_p4est_quad_neighbor_nodes_of_node.h:186-192
+ *   naive_d{x,y,z}_needs_*_correction    src/my_p4est_quad_neighbor_nodes_of_node.h:528-568
+ *   d{x,y,z}_central_internal            src/my_p4est_quad_neighbor_nodes_of_node.cpp:638-760
+ *   inverse_d_max                        src/my_p4est_node_neighbors.cpp:1168-1186
+ *   compute_normals                      src/my_p4est_utils.cpp:1720-1740
+ *   compute_mean_curvature(normals)      src/my_p4est_utils.cpp:1648-1690,
+ *                                        src/my_p4est_quad_neighbor_nodes_of_node.h:2167-2171
  *
  * With one MPI rank every node is a "local" node and the ghost scatters are
  * no-ops, so the layer/local split of the reference collapses to one loop.
@@ -35,6 +44,9 @@
  * analytic roots of interface_Location*), invariants (sign preservation,
  * anchoring, linear signed distance as a fixed point, Neumann mirror walls,
  * second-order convergence of the band error) and known-answer scalars.
+ * First derivatives: exact for quadratic fields on hanging stencils and
+ * (f_p - f_m)/2h on uniform grids; curvature: h*kappa of circles and spheres
+ * converges to h/R and 2h/R (tests/test_oracle_derivatives.py).
  */
 #include <float.h>
 #include <math.h>
@@ -289,7 +301,100 @@ static void ngbd_quadratic(const view_t *v, int64_t n, const double *g, double *
   }
 }
 
+/* ------------------------------------------------------ first derivatives */
+static double central_derivative(double fp, double f0, double fm, double dp, double dm) {
+  return ((fp - f0) * dm / dp + (f0 - fm) * dp / dm) / (dp + dm);
+}
+
+static double dd_correction_weight(double d_m, double d_p, double d_m_m, double d_m_p, double d_p_m, double d_p_p) {
+  return 0.5 * (d_p_m * d_p_p * d_m / d_p - d_m_m * d_m_p * d_p / d_m) / (d_m + d_p);
+}
+
+/* construct_neighbors' inverse_d_max, with its own list of distances */
+static double inverse_d_max(const view_t *v, int64_t n) {
+  double m = MAX2(DF(v, 0, n), DF(v, 1, n));
+  m = MAX2(MAX2(m, DF(v, 2, n)), DF(v, 3, n));
+  if (v->D == 3) m = MAX2(MAX2(m, DF(v, 4, n)), DF(v, 5, n));
+  m = MAX2(MAX2(m, DT(v, 0, 0, n)), DT(v, 0, 1, n)); /* d_m00_m0, d_m00_p0 */
+  m = MAX2(MAX2(m, DT(v, 1, 0, n)), DT(v, 1, 1, n)); /* d_p00_m0, d_p00_p0 */
+  m = MAX2(MAX2(m, DT(v, 2, 0, n)), DT(v, 2, 1, n)); /* d_0m0_m0, d_0m0_p0 */
+  m = MAX2(MAX2(m, DT(v, 3, 0, n)), DT(v, 3, 1, n)); /* d_0p0_m0, d_0p0_p0 */
+  if (v->D == 3) {
+    m = MAX2(MAX2(m, DT(v, 0, 2, n)), DT(v, 0, 3, n)); /* d_m00_0m, d_m00_0p */
+    m = MAX2(MAX2(m, DT(v, 1, 2, n)), DT(v, 1, 3, n)); /* d_p00_0m, d_p00_0p */
+    m = MAX2(MAX2(m, DT(v, 2, 2, n)), DT(v, 2, 3, n)); /* d_0m0_0m, d_0m0_0p */
+    m = MAX2(MAX2(m, DT(v, 3, 2, n)), DT(v, 3, 3, n)); /* d_0p0_0m, d_0p0_0p */
+    m = MAX2(MAX2(m, DT(v, 4, 0, n)), DT(v, 4, 1, n)); /* d_00m_m0, d_00m_p0 */
+    m = MAX2(MAX2(m, DT(v, 4, 2, n)), DT(v, 5, 3, n)); /* d_00m_0m, d_00p_0p (as listed) */
+    m = MAX2(MAX2(m, DT(v, 5, 0, n)), DT(v, 5, 1, n)); /* d_00p_m0, d_00p_p0 */
+    m = MAX2(MAX2(m, DT(v, 5, 2, n)), DT(v, 5, 3, n)); /* d_00p_0m, d_00p_0p */
+  }
+  return 1.0 / m;
+}
+
+/* d/dx_a of g at node n: naive central derivative of the linear face values,
+ * then for each transverse axis (x: y, z; y: x, z; z: x, y) minus its second
+ * derivative times the correction weight where that weight is not zero. */
+static double d_central(const view_t *v, int64_t n, int a, const double *g) {
+  const int fm = 2 * a, fp = 2 * a + 1;
+  double fx = central_derivative(face_linear(v, fp, n, g), g[n], face_linear(v, fm, n, g), DF(v, fp, n), DF(v, fm, n));
+  const double idm = inverse_d_max(v, n);
+  double w[2];
+  int need[2] = {0, 0};
+  int any = 0;
+  for (int k = 0; k < v->D - 1; ++k) {
+    w[k] = dd_correction_weight(DF(v, fm, n), DF(v, fp, n), DT(v, fm, 2 * k, n), DT(v, fm, 2 * k + 1, n),
+                                DT(v, fp, 2 * k, n), DT(v, fp, 2 * k + 1, n));
+    need[k] = !check_if_zero(w[k] * idm);
+    any = any || need[k];
+  }
+  if (any) {
+    double f000, fnb[6], dd[3];
+    laplace(v, n, g, &f000, fnb, dd);
+    static const int transverse[3][2] = {{1, 2}, {0, 2}, {0, 1}};
+    for (int k = 0; k < v->D - 1; ++k)
+      if (need[k]) fx -= dd[transverse[a][k]] * w[k];
+  }
+  return fx;
+}
+
 /* ------------------------------------------------------------ public API */
+int oracle_first_derivatives(const casl_flat_stencil *s, const double *f, double *fd) {
+  view_t v = make_view(s);
+  const int64_t N = v.N;
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n)
+    for (int a = 0; a < v.D; ++a) fd[(size_t)a * N + n] = d_central(&v, n, a, f);
+  return 0;
+}
+
+int oracle_normals(const casl_flat_stencil *s, const double *phi, double *nrm) {
+  view_t v = make_view(s);
+  const int64_t N = v.N;
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n) {
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int a = 0; a < v.D; ++a) g[a] = d_central(&v, n, a, phi);
+    double sq = g[0] * g[0] + g[1] * g[1];
+    if (v.D == 3) sq = sq + g[2] * g[2];
+    const double abs = sqrt(sq);
+    for (int a = 0; a < v.D; ++a) nrm[(size_t)a * N + n] = abs < EPS ? 0.0 : g[a] / abs;
+  }
+  return 0;
+}
+
+int oracle_mean_curvature(const casl_flat_stencil *s, const double *nrm, double *kappa) {
+  view_t v = make_view(s);
+  const int64_t N = v.N;
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n) {
+    double k = d_central(&v, n, 0, nrm) + d_central(&v, n, 1, nrm + N);
+    if (v.D == 3) k = k + d_central(&v, n, 2, nrm + 2 * N);
+    kappa[n] = k;
+  }
+  return 0;
+}
+
 int oracle_second_derivatives(const casl_flat_stencil *s, const double *f, double *fdd) {
   view_t v = make_view(s);
   const int64_t N = v.N;

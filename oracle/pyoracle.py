@@ -47,7 +47,8 @@ def lib():
         h.oracle_interface_location_2nd.restype = d
         h.oracle_interface_location_2nd.argtypes = [d, d, d, d, d, d]
         sp = ctypes.POINTER(L.FlatStencil)
-        for name in ("oracle_second_derivatives", "oracle_ngbd_quadratic", "oracle_face_linear"):
+        for name in ("oracle_second_derivatives", "oracle_ngbd_quadratic", "oracle_face_linear",
+                     "oracle_first_derivatives", "oracle_normals", "oracle_mean_curvature"):
             fn = getattr(h, name)
             fn.restype = ctypes.c_int
             fn.argtypes = [sp, L.c_double_p, L.c_double_p]
@@ -82,6 +83,33 @@ def second_derivatives(tables, f):
     out = np.empty((tables.dim, tables.n_nodes), dtype=np.float64)
     s = tables.struct()
     lib().oracle_second_derivatives(ctypes.byref(s), _dp(f), _dp(out))
+    return out
+
+
+def first_derivatives(tables, f):
+    """[dim][N]: first_derivatives_central."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.empty((tables.dim, tables.n_nodes), dtype=np.float64)
+    s = tables.struct()
+    lib().oracle_first_derivatives(ctypes.byref(s), _dp(f), _dp(out))
+    return out
+
+
+def normals(tables, phi):
+    """[dim][N]: compute_normals."""
+    phi = np.ascontiguousarray(phi, dtype=np.float64)
+    out = np.empty((tables.dim, tables.n_nodes), dtype=np.float64)
+    s = tables.struct()
+    lib().oracle_normals(ctypes.byref(s), _dp(phi), _dp(out))
+    return out
+
+
+def mean_curvature(tables, nrm):
+    """[N]: compute_mean_curvature(normals)."""
+    nrm = np.ascontiguousarray(nrm, dtype=np.float64)
+    out = np.empty(tables.n_nodes, dtype=np.float64)
+    s = tables.struct()
+    lib().oracle_mean_curvature(ctypes.byref(s), _dp(nrm), _dp(out))
     return out
 
 

@@ -235,3 +235,23 @@ def test_batch_graph_replay_and_autotune(cuda):
         phi.copy_(base)
         batch.reinitialize_2nd_order(phi, 6, precision="mixed")
         assert torch.equal(phi, eager)
+
+
+@pytest.mark.parametrize("name", ["star2d", "spheres3d"])
+def test_first_derivatives_normals_curvature_match_oracle(cuda, name):
+    """first_derivatives_central, compute_normals and compute_mean_curvature
+    through the C ABI against the oracle (FP64, same operation order)."""
+    import torch
+    dim, shape, lmin, lmax = PROBLEMS[name]
+    say(f"[gpu] curvature chain {name}")
+    f, ngbd, phi0 = problem(dim, shape, lmin, lmax)
+    tab = ngbd.tables
+    phi = ngbd.device()
+    p = torch.from_numpy(phi0).to(cuda)
+    fd = phi.first_derivatives(p).cpu().numpy()
+    assert rel_err(fd, O.first_derivatives(tab, phi0)) <= REL
+    nrm = C.compute_normals(ngbd, p)
+    nrm_o = O.normals(tab, phi0)
+    assert rel_err(nrm.cpu().numpy(), nrm_o) <= REL
+    kap = C.compute_mean_curvature(ngbd, nrm).cpu().numpy()
+    assert rel_err(kap, O.mean_curvature(tab, nrm_o)) <= REL

@@ -424,6 +424,31 @@ def main():
     eff = dict(mixed=rate_mixed * b_alg(dim, 4) / HBM_PEAK, fp64=rate_fp64 * b_alg(dim, 8) / HBM_PEAK,
                b_alg_mixed=b_alg(dim, 4), b_alg_fp64=b_alg(dim, 8))
 
+    # advect_in_normal_direction on the same grid (FP64): one RK2 step per call
+    import ctypes
+    from casl_p4est_amd import _lib as L
+    _, xyz = W["forest"].nodes()
+    vn = torch.from_numpy(1.0 + 0.5 * np.sin(4 * xyz[:, 0])).to(dev)
+    del xyz
+    adv = phi0.clone()
+    dt_adv = lsr.advect_in_normal_direction(vn, adv)
+    t_adv = time_steps(lambda: lsr.advect_in_normal_direction(vn, adv, dt=dt_adv), args.steps, args.warmup, torch)
+    lib = L.lib()
+    lib.casl_profile_enable(1)
+    lib.casl_profile_reset()
+    lsr.advect_in_normal_direction(vn, adv, dt=dt_adv)
+    torch.cuda.synchronize()
+    adv_k = {}
+    for k in ("k_adv_dd", "k_advect"):
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        L.call("casl_profile_query", k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        adv_k[k] = dict(avg_ms=ms.value / max(1, cnt.value), launches=cnt.value)
+    lib.casl_profile_enable(0)
+    advect = {"value": N * args.steps / t_adv, "unit": "node-steps/s (RK2 step, FP64)",
+              "ms_per_step": 1e3 * t_adv / args.steps, "dt": dt_adv, "kernels": adv_k}
+    log(f"[bench] advect_in_normal_direction: {advect['value']:.3e} node-steps/s, "
+        f"{advect['ms_per_step']:.2f} ms per step")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(W, args.cpu_budget)
@@ -451,6 +476,7 @@ def main():
         "linf_err_band_over_h": err / h,
         "roofline": roofline,
         "effective_hbm_frac": eff,
+        "advect_in_normal_direction": advect,
         "kernels": prof,
         "cpu_baseline": cpu,
     }

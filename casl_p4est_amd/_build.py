@@ -34,7 +34,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", INCLUDE, "-I", CSRC]
 
-HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip", "curv.hip", "derivatives.hip"]
+HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip", "curv.hip", "derivatives.hip", "advect.hip"]
 HOST_SOURCES = ["forest.cpp", "tiles.cpp"]
 
 

@@ -1,0 +1,133 @@
+// advect_in_normal_direction (src/my_p4est_level_set.cpp:559-726, 785-815)
+// and the time step of its velocity-function form (:744-783), FP64 on the
+// grid-wide tables:
+//   phi_n = phi;  dd = second_derivatives_central(phi_n)
+//   phi   = stage(vn,    dd, phi_n)            (advect_in_normal_direction_one_iteration)
+//   dd    = second_derivatives_central(phi)
+//   phi   = (phi_n + stage(vn_np1 or vn, dd, phi)) / 2
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+
+#include "reinit_device.hpp"
+
+using namespace casl;
+using namespace casl::dev;
+
+namespace {
+
+constexpr int BLOCK = 256;
+
+inline unsigned grid_for(int64_t n) { return unsigned((n + BLOCK - 1) / BLOCK); }
+
+template <int D>
+__global__ void __launch_bounds__(BLOCK) k_adv_dd(Tables<D> t, const double *__restrict__ f, double *__restrict__ fdd) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n < t.N) node_dd<D, double>(t, n, f, fdd);
+}
+
+// out = stage, or (RK_AVG) out = (out + stage) / 2 with out holding phi_n
+template <int D, bool RK_AVG>
+__global__ void __launch_bounds__(BLOCK) k_advect(Tables<D> t, const double *__restrict__ vn, double dt,
+                                                  const double *__restrict__ phin, const double *__restrict__ dd,
+                                                  double *out) {
+  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (n >= t.N) return;
+  const double r = advect_value<D>(t, n, t.wall[n], vn[n], dt, phin, dd, t.N);
+  if constexpr (RK_AVG)
+    out[n] = 0.5 * (out[n] + r);
+  else
+    out[n] = r;
+}
+
+// min over nodes of 0.8 |s_min / vn|, s_min the shortest |arm|; non-negative
+// doubles order like their bit patterns, so an unsigned atomicMin reduces them
+template <int D>
+__global__ void __launch_bounds__(BLOCK) k_cfl(Tables<D> t, const double *__restrict__ vn,
+                                               unsigned long long *__restrict__ result) {
+  __shared__ double red[BLOCK];
+  double m = DBL_MAX;
+  for (int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < t.N; n += int64_t(gridDim.x) * blockDim.x) {
+    double s_min = min2(fabs(double(t.d(0, n))), fabs(double(t.d(1, n))));
+    s_min = min2(s_min, min2(fabs(double(t.d(2, n))), fabs(double(t.d(3, n)))));
+    if constexpr (D == 3) s_min = min2(s_min, min2(fabs(double(t.d(4, n))), fabs(double(t.d(5, n)))));
+    m = min2(m, 0.8 * fabs(s_min / vn[n]));
+  }
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int k = BLOCK / 2; k > 0; k >>= 1) {
+    if (int(threadIdx.x) < k) red[threadIdx.x] = min2(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicMin(result, (unsigned long long)__double_as_longlong(red[0]));
+}
+
+__global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
+
+template <int D>
+void run_advect(casl_stencil *st, double *phi, const double *vn, const double *vnp1, double dt, hipStream_t s) {
+  const int64_t N = st->N;
+  const Tables<D> t = st->tables<D>();
+  st->phi0.ensure(N);
+  st->ddn64.ensure(size_t(D) * N);
+  double *phin = st->phi0.p, *dd = st->ddn64.p;
+  const unsigned g = grid_for(N);
+  HIP_CHECK(hipMemcpyAsync(phin, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  prof_begin("k_adv_dd", s);
+  hipLaunchKernelGGL(k_adv_dd<D>, dim3(g), dim3(BLOCK), 0, s, t, phin, dd);
+  prof_end("k_adv_dd", s);
+  prof_begin("k_advect", s);
+  hipLaunchKernelGGL((k_advect<D, false>), dim3(g), dim3(BLOCK), 0, s, t, vn, dt, phin, dd, phi);
+  prof_end("k_advect", s);
+  prof_begin("k_adv_dd", s);
+  hipLaunchKernelGGL(k_adv_dd<D>, dim3(g), dim3(BLOCK), 0, s, t, phi, dd);
+  prof_end("k_adv_dd", s);
+  // phin <- (phi_n + stage(phi)) / 2, then back into phi
+  prof_begin("k_advect", s);
+  hipLaunchKernelGGL((k_advect<D, true>), dim3(g), dim3(BLOCK), 0, s, t, vnp1 ? vnp1 : vn, dt, phi, dd, phin);
+  prof_end("k_advect", s);
+  HIP_CHECK(hipGetLastError());
+  HIP_CHECK(hipMemcpyAsync(phi, phin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int casl_advect_in_normal_direction(casl_stencil_t *st, double *phi, const double *vn, const double *vn_np1,
+                                    double dt, void *stream) {
+  return guarded([&]() {
+    if (!st || !phi || !vn) throw std::invalid_argument("null argument");
+    if (!(dt >= 0.0)) throw std::invalid_argument("time step must be >= 0");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    if (st->dim == 2)
+      run_advect<2>(st, phi, vn, vn_np1, dt, s);
+    else
+      run_advect<3>(st, phi, vn, vn_np1, dt, s);
+  });
+}
+
+int casl_advect_cfl_dt(casl_stencil_t *st, const double *vn, double *dt, void *stream) {
+  return guarded([&]() {
+    if (!st || !vn || !dt) throw std::invalid_argument("null argument");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    st->red.ensure(1);
+    const double big = DBL_MAX;
+    unsigned long long init;
+    std::memcpy(&init, &big, sizeof(init));
+    hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, st->red.p, init);
+    const unsigned g = std::min<unsigned>(grid_for(st->N), 2048u);
+    if (st->dim == 2)
+      hipLaunchKernelGGL(k_cfl<2>, dim3(g), dim3(BLOCK), 0, s, st->tables<2>(), vn, st->red.p);
+    else
+      hipLaunchKernelGGL(k_cfl<3>, dim3(g), dim3(BLOCK), 0, s, st->tables<3>(), vn, st->red.p);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long bits = 0;
+    HIP_CHECK(hipMemcpyAsync(&bits, st->red.p, sizeof(bits), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    std::memcpy(dt, &bits, sizeof(bits));
+  });
+}
+
+}  // extern "C"

@@ -299,6 +299,65 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
   return r;
 }
 
+// One stage of advect_in_normal_direction_one_iteration at one node
+// (src/my_p4est_level_set.cpp:559-652): quadratic neighbor values of phi_n,
+// Neumann mirror on walls, one-sided differences with the ENO/minmod
+// correction, upwinding by the sign of vn, and
+// phi_n - dt vn sqrt(sum of both one-sided squares).  FP64.
+template <int D, class TB>
+__device__ __forceinline__ double advect_value(const TB &t, int64_t j, uint8_t w, double vn, double dt,
+                                               const double *__restrict__ phin, const double *__restrict__ dd,
+                                               int64_t dd_stride) {
+  constexpr int F = 2 * D;
+  double pn_000, pn_f[F], dloc[D];
+  ngbd_quadratic<D, double>(t, j, phin, pn_000, pn_f, dloc);
+  double s[F], dd_000[D], dd_f[F];
+#pragma unroll
+  for (int f = 0; f < F; ++f) s[f] = double(t.d(f, j));
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    dd_000[a] = dd[int64_t(a) * dd_stride + j];
+    dd_f[2 * a] = face_linear<D, double>(t, 2 * a, j, dd + int64_t(a) * dd_stride);
+    dd_f[2 * a + 1] = face_linear<D, double>(t, 2 * a + 1, j, dd + int64_t(a) * dd_stride);
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const int fm = 2 * a, fp = 2 * a + 1;
+    if (w & (1u << fm)) {
+      s[fm] = s[fp];
+      pn_f[fm] = pn_f[fp];
+      dd_000[a] = dd_f[fm] = dd_f[fp] = 0.0;
+    }
+    if (w & (1u << fp)) {
+      s[fp] = s[fm];
+      pn_f[fp] = pn_f[fm];
+      dd_000[a] = dd_f[fm] = dd_f[fp] = 0.0;
+    }
+  }
+  double Dp[D], Dm[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    Dp[a] = (pn_f[2 * a + 1] - pn_000) / s[2 * a + 1];
+    Dm[a] = (pn_000 - pn_f[2 * a]) / s[2 * a];
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    const double ddm = minmod(dd_f[2 * a], dd_000[a]);
+    Dm[a] += 0.5 * s[2 * a] * ddm;
+    const double ddp = minmod(dd_f[2 * a + 1], dd_000[a]);
+    Dp[a] -= 0.5 * s[2 * a + 1] * ddp;
+  }
+  const double sgn = vn > 0.0 ? 1.0 : -1.0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) {
+    if (sgn * Dp[a] > 0.0) Dp[a] = 0.0;
+    if (sgn * Dm[a] < 0.0) Dm[a] = 0.0;
+  }
+  double sum = (Dp[0] * Dp[0] + Dm[0] * Dm[0]) + (Dp[1] * Dp[1] + Dm[1] * Dm[1]);
+  if constexpr (D == 3) sum = sum + (Dp[2] * Dp[2] + Dm[2] * Dm[2]);
+  return pn_000 - dt * vn * sqrt(sum);
+}
+
 // stage_value on the grid-wide tables
 template <int D, class R, bool SECOND_ORDER>
 __device__ __forceinline__ R node_stage(const Tables<D> &t, int64_t n, uint16_t fl, const R *__restrict__ s_fix,

@@ -142,6 +142,30 @@ class DeviceStencil:
                _stream_ptr(_torch()))
         return out.view(-1)
 
+    def advect(self, phi, vn, dt, vn_np1=None):
+        """One advect_in_normal_direction step (FP64, in place)."""
+        self._check_field(phi, "phi")
+        self._check_field(vn, "vn")
+        if vn_np1 is not None:
+            self._check_field(vn_np1, "vn_np1")
+        for t in (phi, vn) + ((vn_np1,) if vn_np1 is not None else ()):
+            if t.numel() != self.n_nodes:
+                raise ValueError("fields must hold one value per node")
+        L.call("casl_advect_in_normal_direction", self._h, ctypes.c_void_p(phi.data_ptr()),
+               ctypes.c_void_p(vn.data_ptr()),
+               ctypes.c_void_p(vn_np1.data_ptr()) if vn_np1 is not None else None, float(dt),
+               _stream_ptr(_torch()))
+        return phi
+
+    def cfl_dt(self, vn):
+        self._check_field(vn, "vn")
+        if vn.numel() != self.n_nodes:
+            raise ValueError("vn must hold one value per node")
+        out = ctypes.c_double()
+        L.call("casl_advect_cfl_dt", self._h, ctypes.c_void_p(vn.data_ptr()), ctypes.byref(out),
+               _stream_ptr(_torch()))
+        return out.value
+
     def band_linf(self, a, b, ref, band):
         torch = _torch()
         for t, n in ((a, "a"), (b, "b"), (ref, "ref")):
@@ -197,6 +221,26 @@ class LevelSet:
             if e.code == L.CASL_ERR_INVALID_ARGUMENT:
                 raise ValueError(str(e)) from e
             raise
+
+    def advect_in_normal_direction(self, vn, phi, dt=None, vn_np1=None):
+        """advect_in_normal_direction (src/my_p4est_level_set.h:625-659): one
+        RK2 step of phi_t + vn |grad phi| = 0 with node-sampled vn (and vn_np1
+        for the second stage).  Without dt the step is the velocity-function
+        form's CFL choice, min 0.8 |s_min / vn|.  phi is updated in place
+        (numpy or float64 CUDA tensor); returns dt."""
+        dev = self.ngbd.device()
+        torch = _torch()
+        if isinstance(phi, np.ndarray):
+            t = torch.from_numpy(np.ascontiguousarray(phi, dtype=np.float64)).cuda()
+            v = torch.from_numpy(np.ascontiguousarray(vn, dtype=np.float64)).cuda()
+            v1 = torch.from_numpy(np.ascontiguousarray(vn_np1, dtype=np.float64)).cuda() if vn_np1 is not None else None
+            dt = dev.cfl_dt(v) if dt is None else dt
+            dev.advect(t, v, dt, v1)
+            phi[:] = t.cpu().numpy()
+            return dt
+        dt = dev.cfl_dt(vn) if dt is None else dt
+        dev.advect(phi, vn, dt, vn_np1)
+        return dt
 
     # --- the public entry points of my_p4est_level_set.h:434-613
     def reinitialize_1st_order(self, phi, number_of_iterations=50, limit_absolute_value_phi_0=DBL_MAX):

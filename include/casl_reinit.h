@@ -126,6 +126,19 @@ int casl_normals(casl_stencil_t *st, const double *phi, double *normals,
 int casl_mean_curvature(casl_stencil_t *st, const double *normals, double *kappa,
                         void *stream);
 
+/* advect_in_normal_direction(phi, dt, vn[, vn_np1]) (src/my_p4est_level_set.cpp:
+ * 559-726, 785-815): one TVD-RK2 step of phi_t + vn |grad phi| = 0 with the
+ * Godunov / ENO-minmod stencil of the reinitialization, the second stage with
+ * vn_np1 when given (may be NULL).  phi, vn, vn_np1: FP64 device arrays. */
+int casl_advect_in_normal_direction(casl_stencil_t *st, double *phi,
+                                    const double *vn, const double *vn_np1,
+                                    double dt, void *stream);
+/* The time step of advect_in_normal_direction(const CF&, phi)
+ * (src/my_p4est_level_set.cpp:744-783): min over nodes of 0.8 |s_min / vn|,
+ * s_min the node's shortest arm.  *dt is host. */
+int casl_advect_cfl_dt(casl_stencil_t *st, const double *vn, double *dt,
+                       void *stream);
+
 /* reinitialize_within_range_of_phi_0(phi, order_space, order_pseudotime,
  * phi_0_limit_high, phi_0_limit_low, number_of_iterations): phi is an FP64
  * device array of n_nodes values, updated in place.  Orders outside {1,2}

@@ -32,6 +32,9 @@
  *   d{x,y,z}_central_internal            src/my_p4est_quad_neighbor_nodes_of_node.cpp:638-760
  *   inverse_d_max                        src/my_p4est_node_neighbors.cpp:1168-1186
  *   compute_normals                      src/my_p4est_utils.cpp:1720-1740
+ *   advect_in_normal_direction(_one_iteration)
+ *                                        src/my_p4est_level_set.cpp:559-726
+ *   its CFL time step (CF form)          src/my_p4est_level_set.cpp:744-783
  *   compute_mean_curvature(normals)      src/my_p4est_utils.cpp:1648-1690,
  *                                        src/my_p4est_quad_neighbor_nodes_of_node.h:2167-2171
  *
@@ -588,6 +591,98 @@ int oracle_reinit(const casl_flat_stencil *s, double *phi, int order_space, int 
 }
 
 /* Threads the oracle's OpenMP loops use (1 when built without OpenMP). */
+/* ---------------------------------------------------------- advection */
+/* advect_in_normal_direction_one_iteration at node n */
+static double advect_one(const view_t *v, int64_t n, const double *vn, double dt, const double *dd,
+                         const double *phi_n) {
+  const int D = v->D;
+  double p000, pf[6], dloc[3];
+  ngbd_quadratic(v, n, phi_n, &p000, pf, dloc);
+  double s[6];
+  for (int f = 0; f < 2 * D; ++f) s[f] = DF(v, f, n);
+  double xx000[3], xxm[3], xxp[3];
+  for (int a = 0; a < D; ++a) {
+    const double *g = dd + (size_t)a * v->N;
+    xx000[a] = g[n];
+    xxm[a] = face_linear(v, 2 * a, n, g);
+    xxp[a] = face_linear(v, 2 * a + 1, n, g);
+  }
+  const uint8_t w = v->wall[n];
+  for (int a = 0; a < D; ++a) {
+    if (w & (1u << (2 * a))) {
+      s[2 * a] = s[2 * a + 1];
+      pf[2 * a] = pf[2 * a + 1];
+      xx000[a] = xxm[a] = xxp[a] = 0.0;
+    }
+    if (w & (1u << (2 * a + 1))) {
+      s[2 * a + 1] = s[2 * a];
+      pf[2 * a + 1] = pf[2 * a];
+      xx000[a] = xxm[a] = xxp[a] = 0.0;
+    }
+  }
+  double dp[3] = {0.0, 0.0, 0.0}, dm[3] = {0.0, 0.0, 0.0};
+  for (int a = 0; a < D; ++a) {
+    dp[a] = (pf[2 * a + 1] - p000) / s[2 * a + 1];
+    dm[a] = (p000 - pf[2 * a]) / s[2 * a];
+  }
+  for (int a = 0; a < D; ++a) {
+    xxm[a] = MINMOD(xxm[a], xx000[a]);
+    dm[a] += 0.5 * s[2 * a] * xxm[a];
+    xxp[a] = MINMOD(xxp[a], xx000[a]);
+    dp[a] -= 0.5 * s[2 * a + 1] * xxp[a];
+  }
+  const double sgn = vn[n] > 0.0 ? +1.0 : -1.0;
+  for (int a = 0; a < D; ++a) {
+    if (sgn * dp[a] > 0.0) dp[a] = 0.0;
+    if (sgn * dm[a] < 0.0) dm[a] = 0.0;
+  }
+  double sum = (dp[0] * dp[0] + dm[0] * dm[0]) + (dp[1] * dp[1] + dm[1] * dm[1]);
+  if (D == 3) sum = sum + (dp[2] * dp[2] + dm[2] * dm[2]);
+  return p000 - dt * vn[n] * (sqrt(sum));
+}
+
+/* advect_in_normal_direction(phi, dt, vn, vn_np1): RK2, phi in place */
+int oracle_advect(const casl_flat_stencil *s, double *phi, const double *vn, const double *vn_np1, double dt) {
+  view_t v = make_view(s);
+  const int64_t N = v.N;
+  double *phi_n = (double *)malloc(sizeof(double) * N);
+  double *phi_np2 = (double *)malloc(sizeof(double) * N);
+  double *dd = (double *)malloc(sizeof(double) * (size_t)v.D * N);
+  if (!phi_n || !phi_np2 || !dd) {
+    free(phi_n);
+    free(phi_np2);
+    free(dd);
+    return -1;
+  }
+  oracle_second_derivatives(s, phi, dd);
+  memcpy(phi_n, phi, sizeof(double) * N);
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n) phi[n] = advect_one(&v, n, vn, dt, dd, phi_n);
+  oracle_second_derivatives(s, phi, dd);
+  const double *v2 = vn_np1 ? vn_np1 : vn;
+#pragma omp parallel for schedule(static)
+  for (int64_t n = 0; n < N; ++n) phi_np2[n] = advect_one(&v, n, v2, dt, dd, phi);
+  for (int64_t n = 0; n < N; ++n) phi[n] = 0.5 * (phi_n[n] + phi_np2[n]);
+  free(phi_n);
+  free(phi_np2);
+  free(dd);
+  return 0;
+}
+
+/* the CF form's time step: min over nodes of 0.8 |s_min / vn| */
+double oracle_advect_cfl_dt(const casl_flat_stencil *s, const double *vn) {
+  view_t v = make_view(s);
+  double dt = DBL_MAX;
+  for (int64_t n = 0; n < v.N; ++n) {
+    double s_min = MIN2(fabs(DF(&v, 0, n)), fabs(DF(&v, 1, n)));
+    s_min = MIN2(s_min, MIN2(fabs(DF(&v, 2, n)), fabs(DF(&v, 3, n))));
+    if (v.D == 3) s_min = MIN2(s_min, MIN2(fabs(DF(&v, 4, n)), fabs(DF(&v, 5, n))));
+    dt = MIN2(dt, 0.8 * fabs(s_min / vn[n]));
+  }
+  return dt;
+}
+
+
 #ifdef _OPENMP
 #include <omp.h>
 int oracle_num_threads(void) { return omp_get_max_threads(); }

@@ -54,6 +54,10 @@ def lib():
             fn.argtypes = [sp, L.c_double_p, L.c_double_p]
         h.oracle_reinit.restype = ctypes.c_int
         h.oracle_reinit.argtypes = [sp, L.c_double_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, d, d, L.c_uint8_p]
+        h.oracle_advect.restype = ctypes.c_int
+        h.oracle_advect.argtypes = [sp, L.c_double_p, L.c_double_p, L.c_double_p, d]
+        h.oracle_advect_cfl_dt.restype = d
+        h.oracle_advect_cfl_dt.argtypes = [sp, L.c_double_p]
         h.oracle_num_threads.restype = ctypes.c_int
         h.oracle_num_threads.argtypes = []
         h.oracle_set_num_threads.restype = None
@@ -111,6 +115,23 @@ def mean_curvature(tables, nrm):
     s = tables.struct()
     lib().oracle_mean_curvature(ctypes.byref(s), _dp(nrm), _dp(out))
     return out
+
+
+def advect(tables, phi, vn, dt, vn_np1=None):
+    """advect_in_normal_direction: one RK2 step, returns the new phi."""
+    out = np.array(phi, dtype=np.float64, copy=True)
+    vn = np.ascontiguousarray(vn, dtype=np.float64)
+    v1 = np.ascontiguousarray(vn_np1, dtype=np.float64) if vn_np1 is not None else None
+    s = tables.struct()
+    if lib().oracle_advect(ctypes.byref(s), _dp(out), _dp(vn), _dp(v1) if v1 is not None else None, dt) != 0:
+        raise MemoryError("oracle_advect")
+    return out
+
+
+def advect_cfl_dt(tables, vn):
+    vn = np.ascontiguousarray(vn, dtype=np.float64)
+    s = tables.struct()
+    return lib().oracle_advect_cfl_dt(ctypes.byref(s), _dp(vn))
 
 
 def ngbd_quadratic(tables, f):

@@ -10,14 +10,14 @@ Level-set functions for refinement and sampling live in .levelset.
 from . import levelset
 from ._lib import CaslError
 from .batch import TreeBatch
-from .forest import FlatStencil, Forest, NodeNeighbors, flat_stencil_from_forest
+from .forest import DeviceGrid, FlatStencil, Forest, NodeNeighbors, flat_stencil_from_forest
 from .level_set import (DBL_MAX, DeviceStencil, LevelSet, compute_mean_curvature, compute_normals,
                         my_p4est_level_set_t)
 
 my_p4est_node_neighbors_t = NodeNeighbors
 
 __all__ = [
-    "CaslError", "DBL_MAX", "DeviceStencil", "FlatStencil", "Forest", "LevelSet", "NodeNeighbors", "TreeBatch",
+    "CaslError", "DBL_MAX", "DeviceGrid", "DeviceStencil", "FlatStencil", "Forest", "LevelSet", "NodeNeighbors", "TreeBatch",
     "compute_mean_curvature", "compute_normals",
     "flat_stencil_from_forest", "levelset", "my_p4est_level_set_t", "my_p4est_node_neighbors_t",
 ]

@@ -8,6 +8,7 @@ writes objects under casl_p4est_amd/_build/.
 import os
 import shutil
 import subprocess
+from concurrent.futures import ThreadPoolExecutor
 import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -34,7 +35,7 @@ HIP_FLAGS = [
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", INCLUDE, "-I", CSRC]
 
-HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip", "curv.hip", "derivatives.hip", "advect.hip"]
+HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip", "curv.hip", "derivatives.hip", "advect.hip", "gpu_forest.hip"]
 HOST_SOURCES = ["forest.cpp", "tiles.cpp"]
 
 
@@ -46,18 +47,18 @@ def _run(cmd, verbose):
 
 def build_library(verbose=True):
     os.makedirs(BUILD, exist_ok=True)
-    objs = []
+    jobs = []
     for src in HOST_SOURCES:
         obj = os.path.join(BUILD, src + ".o")
-        _run([CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append((obj, [CXX, *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
     for src in HIP_SOURCES:
-        path = os.path.join(CSRC, src)
-        if not os.path.exists(path):
-            continue
         obj = os.path.join(BUILD, src + ".o")
-        _run([HIPCC, *HIP_FLAGS, "-c", path, "-o", obj], verbose)
-        objs.append(obj)
+        jobs.append((obj, [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
+    # one compiler per source, as many at once as there are cores
+    workers = max(1, min(len(jobs), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(workers) as ex:
+        list(ex.map(lambda job: _run(job[1], verbose), jobs))
+    objs = [obj for obj, _ in jobs]
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"], verbose)
     os.replace(tmp, LIB)

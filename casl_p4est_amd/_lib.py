@@ -150,6 +150,17 @@ SIGNATURES = {
     "casl_forest_get_nodes": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_double_p]),
     "casl_forest_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LevelSetSpec), c_double_p]),
     "casl_forest_init_neighbors": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FlatStencil)]),
+    "casl_device_grid_build": (ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_double, ctypes.POINTER(LevelSetSpec),
+                                              ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "casl_device_grid_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "casl_device_grid_sizes": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p, c_int32_p]),
+    "casl_device_grid_tables": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FlatStencil)]),
+    "casl_device_grid_download": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_double_p, c_double_p, c_uint8_p,
+                                                  c_int32_p]),
+    "casl_device_grid_sample": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(LevelSetSpec), ctypes.c_void_p,
+                                               ctypes.c_void_p]),
+    "casl_device_grid_stencil": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p)]),
 }
 
 _lib = None

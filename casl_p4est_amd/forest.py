@@ -110,6 +110,102 @@ class Forest:
         return (self.box_max - self.box_min) / float(2 ** ml)
 
 
+class DeviceGrid:
+    """The grid of Forest(dim).refine_levelset(phi, ...).build_nodes() and its
+    construct_neighbors tables, built on the current GPU
+    (casl_device_grid_build): refinement level by level, node numbering by a
+    device radix sort, neighbours one thread per node.  Same leaves, node order
+    and tables as the host Forest whenever phi evaluates to the same bits on
+    both (sphere and plane level sets always do)."""
+
+    def __init__(self, dim, phi, min_lvl, max_lvl, lip=1.2, band=0.0, box_min=None, box_max=None):
+        from .level_set import _stream_ptr, _torch
+        torch = _torch()
+        torch.cuda.init()
+        if dim not in (2, 3):
+            raise ValueError("dim must be 2 or 3")
+        self.dim = dim
+        self.box_min = np.asarray(box_min if box_min is not None else [0.0] * dim, dtype=np.float64)
+        self.box_max = np.asarray(box_max if box_max is not None else [1.0] * dim, dtype=np.float64)
+        h = ctypes.c_void_p()
+        spec = phi.spec()
+        L.call("casl_device_grid_build", dim, self.box_min.ctypes.data_as(L.c_double_p),
+               self.box_max.ctypes.data_as(L.c_double_p), int(min_lvl), int(max_lvl), float(lip), float(band),
+               ctypes.byref(spec), _stream_ptr(torch), ctypes.byref(h))
+        self._h = h
+        self._device = None
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._device = None
+            L.lib().casl_device_grid_destroy(h)
+            self._h = None
+
+    def sizes(self):
+        nl, nn, ml = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()
+        L.call("casl_device_grid_sizes", self._h, ctypes.byref(nl), ctypes.byref(nn), ctypes.byref(ml))
+        return nl.value, nn.value, ml.value
+
+    @property
+    def num_leaves(self):
+        return self.sizes()[0]
+
+    @property
+    def num_nodes(self):
+        return self.sizes()[1]
+
+    @property
+    def max_level(self):
+        return self.sizes()[2]
+
+    def dxyz_min(self):
+        return (self.box_max - self.box_min) / float(2 ** self.max_level)
+
+    def tables(self):
+        """The tables copied to the host, as a FlatStencil."""
+        n, D = self.num_nodes, self.dim
+        F, C, T = 2 * D, 2 ** (D - 1), 2 * (D - 1)
+        nbr = np.empty((F, C, n), dtype=np.int32)
+        d = np.empty((F, n), dtype=np.float64)
+        d_tr = np.empty((F, T, n), dtype=np.float64)
+        wall = np.empty(n, dtype=np.uint8)
+        L.call("casl_device_grid_download", self._h, nbr.ctypes.data_as(L.c_int32_p),
+               d.ctypes.data_as(L.c_double_p), d_tr.ctypes.data_as(L.c_double_p),
+               wall.ctypes.data_as(L.c_uint8_p), None)
+        s = L.FlatStencil()
+        L.call("casl_device_grid_tables", self._h, ctypes.byref(s))
+        return FlatStencil(D, nbr, d, d_tr, wall, s.zero_distance_threshold)
+
+    def node_ixyz(self):
+        """Unclamped logical node coordinates [N][dim] (as Forest.nodes()[0])."""
+        ixyz = np.empty((self.num_nodes, self.dim), dtype=np.int32)
+        L.call("casl_device_grid_download", self._h, None, None, None, None, ixyz.ctypes.data_as(L.c_int32_p))
+        return ixyz
+
+    def sample(self, phi, out=None):
+        """sample_cf_on_nodes on the device: a float64 tensor on the current GPU."""
+        from .level_set import _stream_ptr, _torch
+        torch = _torch()
+        if out is None:
+            out = torch.empty(self.num_nodes, dtype=torch.float64, device="cuda")
+        if out.dtype != torch.float64 or not out.is_cuda or out.numel() != self.num_nodes or not out.is_contiguous():
+            raise ValueError("out must be a contiguous float64 CUDA tensor with one value per node")
+        spec = phi.spec()
+        L.call("casl_device_grid_sample", self._h, ctypes.byref(spec), ctypes.c_void_p(out.data_ptr()),
+               _stream_ptr(torch))
+        return out
+
+    def device(self):
+        """A DeviceStencil for this grid's tables (casl_device_grid_stencil), created once."""
+        if self._device is None:
+            from .level_set import DeviceStencil
+            h = ctypes.c_void_p()
+            L.call("casl_device_grid_stencil", self._h, ctypes.byref(h))
+            self._device = DeviceStencil(None, handle=h)
+        return self._device
+
+
 class FlatStencil:
     """Flattened quad_neighbor_nodes_of_node_t tables (numpy views, SoA).
 

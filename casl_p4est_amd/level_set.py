@@ -34,10 +34,15 @@ def _stream_ptr(torch):
 class DeviceStencil:
     """Device copy of the flattened stencils (casl_stencil_create)."""
 
-    def __init__(self, tables):
+    def __init__(self, tables, handle=None):
         torch = _torch()
         torch.cuda.init()
         self.tables = tables
+        if handle is not None:  # a handle created by the library (casl_device_grid_stencil)
+            dim, n = ctypes.c_int32(), ctypes.c_int64()
+            L.call("casl_stencil_info", handle, ctypes.byref(dim), ctypes.byref(n))
+            self.dim, self.n_nodes, self._h = dim.value, n.value, handle
+            return
         self.dim = tables.dim
         self.n_nodes = tables.n_nodes
         h = ctypes.c_void_p()

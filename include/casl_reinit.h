@@ -330,6 +330,42 @@ int casl_forest_sample(const casl_forest_t *f, const casl_levelset *phi,
  * stay valid until the forest is destroyed. */
 int casl_forest_init_neighbors(casl_forest_t *f, casl_flat_stencil *out);
 
+/* ------------------------------------------ grid construction on the device
+ * The same grid as casl_forest_new + casl_forest_refine_levelset +
+ * casl_forest_build_nodes + casl_forest_init_neighbors, built on the current
+ * device: refine_levelset_cf level by level (my_p4est_refine_coarsen.cpp:17-89),
+ * p4est's node numbering (my_p4est_nodes.c:383-446, 614-671) and
+ * construct_neighbors (my_p4est_node_neighbors.cpp:114-1201).  The tables,
+ * node order and node coordinates equal the host forest's whenever the level
+ * set evaluates to the same bits on host and device (always for SPHERE and
+ * PLANE; sin, cos and atan2 may differ in the last place, which can move a
+ * refinement decision taken exactly at its threshold).  CALLBACK level sets
+ * are host-only (CASL_ERR_INVALID).  Synchronous on `stream`. */
+typedef struct casl_device_grid casl_device_grid_t;
+int casl_device_grid_build(int dim, const double *box_min,
+                           const double *box_max, int min_lvl, int max_lvl,
+                           double lip, double band, const casl_levelset *phi,
+                           void *stream, casl_device_grid_t **out);
+int casl_device_grid_destroy(casl_device_grid_t *g);
+int casl_device_grid_sizes(const casl_device_grid_t *g, int64_t *n_leaves,
+                           int64_t *n_nodes, int32_t *max_level);
+/* Device pointers to the tables, laid out as casl_flat_stencil describes;
+ * valid until the grid is destroyed. */
+int casl_device_grid_tables(const casl_device_grid_t *g,
+                            casl_flat_stencil *out);
+/* Copies the tables (and the unclamped logical node coordinates, [N][dim])
+ * to host arrays; any pointer may be NULL to skip that array. */
+int casl_device_grid_download(const casl_device_grid_t *g, int32_t *nbr,
+                              double *d, double *d_tr, uint8_t *wall,
+                              int32_t *ixyz);
+/* sample_cf_on_nodes on the device: phi[n] (device FP64) at every node. */
+int casl_device_grid_sample(const casl_device_grid_t *g,
+                            const casl_levelset *phi, double *out,
+                            void *stream);
+/* A device stencil handle (as casl_stencil_create) for the grid's tables. */
+int casl_device_grid_stencil(const casl_device_grid_t *g,
+                             casl_stencil_t **out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,142 @@
+"""Grid construction on the device (casl_device_grid_*) against the host
+forest (Forest.refine_levelset + build_nodes + construct_neighbors):
+
+  * the same number of leaves, the same finest level, the same nodes in the
+    same order (unclamped logical coordinates);
+  * the construct_neighbors tables (neighbour ids, face and transverse
+    distances, wall bits, zero-distance threshold) bit for bit;
+  * sample_cf_on_nodes: bit for bit for the sphere and plane level sets
+    (correctly rounded arithmetic only), within 4 ulp of |phi| scale for the
+    level sets that call sin, cos or atan2;
+  * reinitialize_2nd_order on the device-built stencil equals the host-built
+    one bit for bit;
+  * the level sets refine the measured workloads (star2d, spheres3d at full
+    size, gyroid3d at lmax 9) to the same grids.
+"""
+import time
+
+import numpy as np
+import pytest
+
+import casl_p4est_amd as C
+from casl_p4est_amd import levelset as ls
+from casl_p4est_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+
+
+def host_grid(dim, phi, lmin, lmax, lip, band, box_min=None, box_max=None):
+    f = C.Forest(dim, box_min, box_max).refine_levelset(phi, lmin, lmax, lip=lip, band=band).build_nodes()
+    return f, C.flat_stencil_from_forest(f)
+
+
+def assert_same_grid(f, tab, g):
+    assert g.sizes() == f.sizes()
+    assert np.array_equal(g.node_ixyz(), f.nodes()[0])
+    dt = g.tables()
+    assert dt.zero_distance_threshold == tab.zero_distance_threshold
+    assert np.array_equal(dt.nbr, tab.nbr)
+    assert np.array_equal(dt.d.view(np.uint64), tab.d.view(np.uint64))
+    assert np.array_equal(dt.d_tr.view(np.uint64), tab.d_tr.view(np.uint64))
+    assert np.array_equal(dt.wall, tab.wall)
+
+
+EXACT = [
+    # name, dim, level set, lmin, lmax, lip, band, box_min, box_max
+    ("circle2d", 2, ls.Sphere([0.5, 0.5], 0.3), 2, 8, 1.2, 0.0, None, None),
+    ("circle2d_box", 2, ls.Sphere([0.1, -0.2], 0.45), 1, 7, 2.4, 1.0, [-1.0, -1.0], [1.0, 1.0]),
+    ("circle2d_offcentre_box", 2, ls.Sphere([0.0, 0.0], 0.7), 3, 9, 1.2, 0.0, [-0.5, -1.0], [1.5, 1.0]),
+    ("plane2d", 2, ls.Plane([0.6, -0.8], 0.05), 2, 10, 1.2, 0.0, None, None),
+    ("uniform2d", 2, ls.Sphere([0.5, 0.5], 0.3), 5, 5, 1.2, 0.0, None, None),
+    ("lmin_above_lmax2d", 2, ls.Sphere([0.5, 0.5], 0.3), 6, 3, 1.2, 0.0, None, None),
+    ("sphere3d", 3, ls.Sphere([0.45, 0.5, 0.55], 0.3), 2, 7, 1.2, 0.0, None, None),
+    ("sphere3d_box", 3, ls.Sphere([0.2, 0.1, -0.1], 0.6), 1, 6, 2.4, 0.0, [-1.0, -1.0, -1.0], [1.0, 1.0, 1.0]),
+    ("plane3d", 3, ls.Plane([0.48, 0.6, -0.64], -0.1), 1, 7, 1.2, 0.0, None, None),
+    ("spheres3d_unmodulated", 3, ls.Spheres([[0.3, 0.3, 0.3], [0.62, 0.55, 0.6]], [0.12, 0.2], 0.0),
+     2, 7, 2.4, 0.0, None, None),
+]
+
+
+@pytest.mark.parametrize("case", EXACT, ids=[c[0] for c in EXACT])
+def test_device_grid_equals_host(cuda, case):
+    import torch
+    name, dim, phi, lmin, lmax, lip, band, bmin, bmax = case
+    say(f"[grid] {name}")
+    f, tab = host_grid(dim, phi, lmin, lmax, lip, band, bmin, bmax)
+    g = C.DeviceGrid(dim, phi, lmin, lmax, lip=lip, band=band, box_min=bmin, box_max=bmax)
+    assert_same_grid(f, tab, g)
+    hs = f.sample(phi)
+    ds = g.sample(phi).cpu().numpy()
+    assert np.array_equal(ds.view(np.uint64), hs.view(np.uint64))
+    # the device-built stencil runs the same reinitialization
+    a = torch.from_numpy(hs).to(cuda)
+    b = a.clone()
+    C.DeviceStencil(tab).reinit(a, 2, 2, 5, C.DBL_MAX, -C.DBL_MAX, 0)
+    g.device().reinit(b, 2, 2, 5, C.DBL_MAX, -C.DBL_MAX, 0)
+    assert torch.equal(a, b)
+
+
+def test_device_grid_root_only_fails_like_host(cuda):
+    """A grid of the root alone has a wall on both sides of every node: the
+    host and the device construction both refuse it."""
+    phi = ls.Sphere([0.5, 0.5], 10.0)  # far from every corner: no refinement
+    f = C.Forest(2).refine_levelset(phi, 0, 4, lip=1.2).build_nodes()
+    with pytest.raises(C.CaslError, match="second-degree"):
+        C.flat_stencil_from_forest(f)
+    with pytest.raises(C.CaslError, match="second-degree"):
+        C.DeviceGrid(2, phi, 0, 4, lip=1.2)
+
+
+def test_device_grid_rejects_bad_arguments(cuda):
+    phi = ls.Sphere([0.5, 0.5], 0.3)
+    with pytest.raises(C.CaslError):
+        C.DeviceGrid(2, phi, 2, 29)  # beyond P4EST_MAXLEVEL - 2 in 2D
+    with pytest.raises(C.CaslError):
+        C.DeviceGrid(3, ls.Sphere([0.5, 0.5, 0.5], 0.3), 2, 18)
+    with pytest.raises(C.CaslError):
+        C.DeviceGrid(2, ls.Gyroid(), 2, 6)  # a 3D level set
+    with pytest.raises(C.CaslError):
+        C.DeviceGrid(2, ls.Callback(lambda x: x[0], 2), 2, 6)  # host callbacks stay on the host
+    with pytest.raises(C.CaslError):
+        C.DeviceGrid(2, phi, 2, 6, box_min=[0.0, 0.0], box_max=[1.0, 0.0])
+
+
+TRANSCENDENTAL = [
+    ("modulated2d", 2, ls.Modulated((0.0, 0.0), 0.5, 0.5), 3, 9, 1.2, 0.0, [-1.0, -1.0], [1.0, 1.0]),
+    ("star2d_small", 2, ls.Star(), 4, 8, 2.4, 0.0, None, None),
+    ("spheres3d_small", 3, ls.random_spheres(16, seed=7), 3, 7, 2.4, 0.0, None, None),
+    ("gyroid3d_small", 3, ls.Gyroid(), 3, 7, 2.4, 0.0, None, None),
+]
+
+
+@pytest.mark.parametrize("case", TRANSCENDENTAL, ids=[c[0] for c in TRANSCENDENTAL])
+def test_device_grid_transcendental_level_sets(cuda, case):
+    name, dim, phi, lmin, lmax, lip, band, bmin, bmax = case
+    say(f"[grid] {name}")
+    f, tab = host_grid(dim, phi, lmin, lmax, lip, band, bmin, bmax)
+    g = C.DeviceGrid(dim, phi, lmin, lmax, lip=lip, band=band, box_min=bmin, box_max=bmax)
+    assert_same_grid(f, tab, g)
+    hs = f.sample(phi)
+    ds = g.sample(phi).cpu().numpy()
+    scale = float(np.max(np.abs(hs)))
+    assert float(np.max(np.abs(ds - hs))) <= 4 * np.finfo(np.float64).eps * scale
+
+
+@pytest.mark.parametrize("name,lmax", [("star2d", None), ("spheres3d", None), ("gyroid3d", 9)])
+def test_device_grid_workloads(cuda, name, lmax):
+    cfg = dict(workloads.TREES[name])
+    if lmax:
+        cfg["lmax"] = lmax
+    phi = workloads.tree_shape(name)
+    say(f"[grid] workload {name}, lmin {cfg['lmin']} / lmax {cfg['lmax']}")
+    t0 = time.time()
+    f, tab = host_grid(cfg["dim"], phi, cfg["lmin"], cfg["lmax"], 2.4, 0.0)
+    t1 = time.time()
+    g = C.DeviceGrid(cfg["dim"], phi, cfg["lmin"], cfg["lmax"], lip=2.4)
+    t2 = time.time()
+    say(f"    {f.num_nodes} nodes: host {t1 - t0:.2f} s, device {t2 - t1:.2f} s")
+    assert_same_grid(f, tab, g)

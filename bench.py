@@ -54,6 +54,15 @@ def dd_bytes(dim, w):
     return w + dim * w + 8 * dim
 
 
+def workload_threads():
+    """Threads the host forest construction uses (csrc/forest.cpp num_threads)."""
+    for k in ("CASL_NUM_THREADS", "OMP_NUM_THREADS"):
+        v = os.environ.get(k)
+        if v and v.isdigit() and int(v) > 0:
+            return min(int(v), 64)
+    return min(os.cpu_count() or 1, 64)
+
+
 def build_workload(name, lmax_override=None):
     from casl_p4est_amd import workloads
     return workloads.build_tree(name, lmax_override, log=log)
@@ -449,6 +458,28 @@ def main():
     log(f"[bench] advect_in_normal_direction: {advect['value']:.3e} node-steps/s, "
         f"{advect['ms_per_step']:.2f} ms per step")
 
+    # the same grid built on the device (refinement, node numbering, construct_neighbors)
+    def build_on_device():
+        g = C.DeviceGrid(dim, W["shape"], W["lmin"], W["lmax"], lip=2.4)
+        torch.cuda.synchronize()
+        return g
+    g = build_on_device()
+    if g.sizes() != W["forest"].sizes():
+        raise RuntimeError("device-built grid differs from the host-built one")
+    del g
+    t_grid = []
+    for _ in range(max(1, args.steps)):
+        t0 = time.perf_counter()
+        g = build_on_device()
+        t_grid.append(time.perf_counter() - t0)
+        del g
+    host_s = sum(W["build_s"].values())
+    grid = {"device_s": min(t_grid), "host_s": host_s, "host_phases_s": W["build_s"],
+            "host_threads": workload_threads(), "speedup": host_s / min(t_grid),
+            "note": "refine_levelset_cf + node numbering + construct_neighbors, tables left in HBM"}
+    log(f"[bench] grid construction: device {grid['device_s']:.3f} s, host {host_s:.2f} s "
+        f"({grid['host_threads']} threads), x{grid['speedup']:.1f}")
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(W, args.cpu_budget)
@@ -477,6 +508,7 @@ def main():
         "roofline": roofline,
         "effective_hbm_frac": eff,
         "advect_in_normal_direction": advect,
+        "grid_construction": grid,
         "kernels": prof,
         "cpu_baseline": cpu,
     }

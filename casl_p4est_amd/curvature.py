@@ -16,6 +16,20 @@ the library's own kernels:
     train                   mini-batch Adam on normalised inputs
     hybrid_hk               casl_curvature_normalize -> network -> casl_curvature_hybrid
 
+In 3D (no counterpart in the reference, which is 2D only) the inputs come
+from any grid through the stencil tables (casl_curvature_grid_samples): the
+3x3x3 phi/h block around every node with |phi| < band plus the interpolated
+numerical h*kappa, put in one canonical orientation of the cube
+(casl_curvature_normalize_3d), for a 28-128-128-128-128-1 network:
+
+    UniformBox              a (2^L)^3 uniform grid on [0, size]^3 as one tree
+    grid_samples            the samples of any DeviceStencil
+    make_dataset_3d         spheres and wave surfaces w = A sin(w u + a) sin(w v + b)
+                            in random frames -> phi0 -> reinitialize_2nd_order
+                            (FP64, 20 iterations) -> grid_samples, exact h*kappa
+                            (h * div n) at the closest surface point
+    gyroid_closest_hk       exact h*kappa of the gyroid level set
+
 The network predicts a correction to the numerical value (residual=True):
 h*kappa = -|ihk| + net(x) for a normalised input x; the hybrid estimate uses
 the numerical h*kappa alone where |ihk| < 0.004 and blends up to 0.007, as the
@@ -35,6 +49,7 @@ from .levelset import Plane
 LO_MIN_HK = 0.004
 UP_MIN_HK = 0.007
 DIMS_2D = (10, 128, 128, 128, 128, 1)
+DIMS_3D = (28, 128, 128, 128, 128, 1)
 
 
 def _ptr(t):
@@ -300,15 +315,17 @@ def hybrid(feat, pred, lo=LO_MIN_HK, hi=UP_MIN_HK, residual=True):
 
 
 def training_pairs(feat, target):
-    """Normalised inputs [2n][10] and their targets: the exact h*kappa with
-    the sign convention of the normalised stencil, minus the normalised
-    numerical value -|ihk| (the network learns the correction)."""
+    """Normalised inputs ([2n][10] in 2D, [n][28] in 3D) and their targets:
+    the exact h*kappa with the sign convention of the normalised stencil,
+    minus the normalised numerical value -|ihk| (the network learns the
+    correction)."""
     torch = _torch()
-    x = normalize(feat)
+    three = feat.shape[1] == 28
+    x = normalize_3d(feat) if three else normalize(feat)
     t = torch.as_tensor(target, dtype=torch.float64, device=feat.device)
-    ihk = feat[:, 9].double()
-    tn = torch.where(ihk > 0, -t, t) + ihk.abs()
-    return x, tn.float().repeat_interleave(2).contiguous()
+    ihk = feat[:, -1].double()
+    tn = (torch.where(ihk > 0, -t, t) + ihk.abs()).float()
+    return x, (tn if three else tn.repeat_interleave(2)).contiguous()
 
 
 def train(net, feat, target, epochs=10, batch=16384, lr=1e-3, lr_final=1e-5, seed=0, log=None):
@@ -353,7 +370,238 @@ def train(net, feat, target, epochs=10, batch=16384, lr=1e-3, lr_final=1e-5, see
 
 
 def hybrid_hk(net, feat, residual=True):
-    """Learned h*kappa of samples feat [n][10] (device)."""
+    """Learned h*kappa of samples feat ([n][10] or [n][28], device)."""
+    if feat.shape[1] == 28:
+        return hybrid_3d(feat, net.forward(normalize_3d(feat)), residual=residual)
     x = normalize(feat)
     y = net.forward(x)
     return hybrid(feat, y, residual=residual)
+
+
+# ------------------------------------------------------------------ 3D
+def normalize_3d(feat):
+    """casl_curvature_normalize_3d: [n][28] samples -> [n][28] network inputs."""
+    torch = _torch()
+    out = torch.empty_like(feat)
+    L.call("casl_curvature_normalize_3d", _ptr(feat), int(feat.shape[0]), _ptr(out), _stream_ptr(torch))
+    return out
+
+
+def hybrid_3d(feat, pred, lo=LO_MIN_HK, hi=UP_MIN_HK, residual=True):
+    torch = _torch()
+    hk = torch.empty(feat.shape[0], dtype=torch.float32, device=feat.device)
+    L.call("casl_curvature_hybrid_3d", _ptr(feat), _ptr(pred), int(feat.shape[0]), float(lo), float(hi),
+           int(bool(residual)), _ptr(hk), _stream_ptr(torch))
+    return hk
+
+
+def grid_samples(ds, phi, h, band, normals=None, kappa=None):
+    """casl_curvature_grid_samples on a DeviceStencil: every node with
+    |phi| < band that has a uniform 3^dim block at spacing h.  Returns
+    (feat [n][3^dim+1] float32, offset [n][dim] float64, node [n] int64) on the
+    device, sorted by node, and the count of band nodes without a block."""
+    torch = _torch()
+    if normals is None:
+        normals = ds.normals(phi)
+    if kappa is None:
+        kappa = ds.mean_curvature(normals)
+    nf = 3 ** ds.dim + 1
+    cap = max(1024, ds.n_nodes // 8)
+    while True:
+        feat = torch.empty((cap, nf), dtype=torch.float32, device=phi.device)
+        off = torch.empty((cap, ds.dim), dtype=torch.float64, device=phi.device)
+        node = torch.empty(cap, dtype=torch.int64, device=phi.device)
+        cnt, skipped = ctypes.c_int64(), ctypes.c_int64()
+        L.call("casl_curvature_grid_samples", ds.handle, _ptr(phi), _ptr(normals), _ptr(kappa), float(h),
+               float(band), _ptr(feat), _ptr(off), _ptr(node), cap, ctypes.byref(cnt), ctypes.byref(skipped),
+               _stream_ptr(torch))
+        if cnt.value <= cap:
+            break
+        cap = cnt.value
+    n = cnt.value
+    order = torch.argsort(node[:n])
+    return feat[:n][order], off[:n][order], node[:n][order], skipped.value
+
+
+class UniformBox:
+    """Uniform grid of 2^level cells per side on [0, size]^3 (one tree refined
+    to min = max level), with its device stencil and solver."""
+
+    def __init__(self, level=7, size=1.0):
+        self.level, self.size = level, float(size)
+        self.M = 2 ** level
+        self.h = self.size / self.M
+        f = Forest(3, [0.0] * 3, [self.size] * 3).refine_levelset(Plane([1.0, 0.0, 0.0], -0.5 * self.size),
+                                                                level, level)
+        f.build_nodes()
+        self.forest = f
+        self.ngbd = NodeNeighbors(f).init_neighbors()
+        self.n_nodes = self.ngbd.num_nodes
+        if self.n_nodes != (self.M + 1) ** 3:
+            raise RuntimeError("uniform box has an unexpected node count")
+        _, self.xyz = f.nodes()
+        self.ds = self.ngbd.device()
+        self.solver = LevelSet(self.ngbd)
+
+
+def random_frame(rng):
+    """A random rotation (columns e_u, e_v, e_w) from the QR of a Gaussian matrix."""
+    q, r = np.linalg.qr(rng.standard_normal((3, 3)))
+    return q * np.sign(np.diag(r))
+
+
+def sphere3_phi0(xyz, c, R):
+    """(|x - c|^2 - R^2) / (2 R): negative inside, not a signed distance."""
+    d2 = ((xyz - np.asarray(c)) ** 2).sum(axis=1)
+    return (d2 - R * R) / (2.0 * R)
+
+
+def wave_phi0(xyz, c, Q, A, omega, a, b):
+    """w - A sin(omega u + a) sin(omega v + b) in the frame Q about c."""
+    uvw = (xyz - np.asarray(c)) @ Q
+    return uvw[:, 2] - A * np.sin(omega * uvw[:, 0] + a) * np.sin(omega * uvw[:, 1] + b)
+
+
+def wave_closest_hk(points, c, Q, A, omega, a, b, h):
+    """Exact h * div(n) at the surface point closest to each point (Newton on
+    the surface parameters (u, v), started at the point's own (u, v))."""
+    p = (np.asarray(points) - np.asarray(c)) @ Q
+    u, v = p[:, 0].copy(), p[:, 1].copy()
+
+    def derivs(u, v):
+        su, cu = np.sin(omega * u + a), np.cos(omega * u + a)
+        sv, cv = np.sin(omega * v + b), np.cos(omega * v + b)
+        f = A * su * sv
+        fu, fv = A * omega * cu * sv, A * omega * su * cv
+        fuu, fvv = -A * omega * omega * su * sv, -A * omega * omega * su * sv
+        fuv = A * omega * omega * cu * cv
+        return f, fu, fv, fuu, fvv, fuv
+
+    for _ in range(12):
+        f, fu, fv, fuu, fvv, fuv = derivs(u, v)
+        r = f - p[:, 2]
+        g1 = (u - p[:, 0]) + r * fu
+        g2 = (v - p[:, 1]) + r * fv
+        h11 = 1.0 + fu * fu + r * fuu
+        h22 = 1.0 + fv * fv + r * fvv
+        h12 = fu * fv + r * fuv
+        det = h11 * h22 - h12 * h12
+        ok = det > 1e-3
+        du = np.where(ok, (h22 * g1 - h12 * g2) / np.where(ok, det, 1.0), 0.1 * g1)
+        dv = np.where(ok, (h11 * g2 - h12 * g1) / np.where(ok, det, 1.0), 0.1 * g2)
+        u, v = u - du, v - dv
+    f, fu, fv, fuu, fvv, fuv = derivs(u, v)
+    s2 = 1.0 + fu * fu + fv * fv
+    return h * -((1.0 + fv * fv) * fuu - 2.0 * fu * fv * fuv + (1.0 + fu * fu) * fvv) / s2 ** 1.5
+
+
+def implicit_closest_hk(phi_fn, grad_hess_fn, points, h, iterations=6):
+    """Exact h * div(grad phi / |grad phi|) at the point of {phi = 0} reached
+    from each point by Newton steps along the gradient (the foot point for
+    points this close to the surface)."""
+    x = np.array(points, dtype=np.float64)
+    for _ in range(iterations):
+        g, _ = grad_hess_fn(x)
+        x = x - (phi_fn(x) / (g * g).sum(axis=1))[:, None] * g
+    g, H = grad_hess_fn(x)
+    gn2 = (g * g).sum(axis=1)
+    tr = H[:, 0, 0] + H[:, 1, 1] + H[:, 2, 2]
+    gHg = np.einsum("ni,nij,nj->n", g, H, g)
+    return h * (gn2 * tr - gHg) / gn2 ** 1.5
+
+
+def gyroid_fields(w=8 * math.pi, c=0.3, e=0.5):
+    """phi(x) = G(x) (1 + e x y z), G = sin(wx)cos(wy) + sin(wy)cos(wz) + sin(wz)cos(wx) - c,
+    with its gradient and Hessian (the level set of levelset.Gyroid)."""
+
+    def parts(x):
+        X, Y, Z = x[:, 0], x[:, 1], x[:, 2]
+        sx, cx, sy, cy, sz, cz = (np.sin(w * X), np.cos(w * X), np.sin(w * Y), np.cos(w * Y),
+                                  np.sin(w * Z), np.cos(w * Z))
+        G = sx * cy + sy * cz + sz * cx - c
+        Gx = w * (cx * cy - sz * sx)
+        Gy = w * (-sx * sy + cy * cz)
+        Gz = w * (-sy * sz + cz * cx)
+        Gxx = w * w * (-sx * cy - sz * cx)
+        Gyy = w * w * (-sx * cy - sy * cz)
+        Gzz = w * w * (-sy * cz - sz * cx)
+        Gxy = -w * w * cx * sy
+        Gyz = -w * w * cy * sz
+        Gxz = -w * w * cz * sx
+        m = 1.0 + e * X * Y * Z
+        mx, my, mz = e * Y * Z, e * X * Z, e * X * Y
+        return G, (Gx, Gy, Gz), (Gxx, Gyy, Gzz, Gxy, Gyz, Gxz), m, (mx, my, mz), (e * Z, e * X, e * Y)
+
+    def phi(x):
+        G, _, _, m, _, _ = parts(x)
+        return G * m
+
+    def grad_hess(x):
+        G, (Gx, Gy, Gz), (Gxx, Gyy, Gzz, Gxy, Gyz, Gxz), m, (mx, my, mz), (mxy, myz, mxz) = parts(x)
+        g = np.stack([Gx * m + G * mx, Gy * m + G * my, Gz * m + G * mz], axis=1)
+        H = np.empty((x.shape[0], 3, 3))
+        H[:, 0, 0] = Gxx * m + 2 * Gx * mx
+        H[:, 1, 1] = Gyy * m + 2 * Gy * my
+        H[:, 2, 2] = Gzz * m + 2 * Gz * mz
+        H[:, 0, 1] = H[:, 1, 0] = Gxy * m + Gx * my + Gy * mx + G * mxy
+        H[:, 1, 2] = H[:, 2, 1] = Gyz * m + Gy * mz + Gz * my + G * myz
+        H[:, 0, 2] = H[:, 2, 0] = Gxz * m + Gx * mz + Gz * mx + G * mxz
+        return g, H
+
+    return phi, grad_hess
+
+
+def make_dataset_3d(box, n_spheres, n_waves, seed=0, iterations=20, per_shape=4096, log=None):
+    """Samples near random spheres (radius log-uniform in [2h, 8], the surface
+    passing within 0.3 of the box centre) and wave surfaces (A U[0.01, 0.2],
+    omega U[2 pi, 12 pi], random phases, frame and offset), each phi0
+    reinitialized by the GPU solver (FP64), at most per_shape samples per
+    shape.  Returns dict(feat (device float32 [n][28]), target (numpy exact
+    h*kappa), kind (0 sphere, 1 wave), timings)."""
+    torch = _torch()
+    rng = np.random.default_rng(seed)
+    h, mid = box.h, 0.5 * box.size
+    shapes = []
+    for _ in range(n_spheres):
+        R = float(np.exp(rng.uniform(np.log(2 * h), np.log(8.0))))
+        u = rng.standard_normal(3)
+        u /= np.linalg.norm(u)
+        d = max(0.0, R - 0.3 * rng.uniform(0.3, 1.0))
+        c = mid - d * u + rng.uniform(-h / 2, h / 2, 3)
+        shapes.append(("sphere", c, R))
+    for _ in range(n_waves):
+        c = mid + rng.uniform(-h / 2, h / 2, 3)
+        shapes.append(("wave", c, random_frame(rng), rng.uniform(0.01, 0.2), rng.uniform(2 * math.pi, 12 * math.pi),
+                       rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)))
+    feats, targets, kinds = [], [], []
+    t_reinit = t_samples = 0.0
+    phi = torch.empty(box.n_nodes, dtype=torch.float64, device="cuda")
+    for k, s in enumerate(shapes):
+        p0 = sphere3_phi0(box.xyz, s[1], s[2]) if s[0] == "sphere" else wave_phi0(box.xyz, *s[1:])
+        phi.copy_(torch.from_numpy(p0))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        box.solver.reinitialize_2nd_order(phi, iterations, precision="fp64")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        feat, off, node, _ = grid_samples(box.ds, phi, h, 1.5 * h)
+        torch.cuda.synchronize()
+        t_reinit += t1 - t0
+        t_samples += time.perf_counter() - t1
+        if feat.shape[0] == 0:
+            continue
+        if feat.shape[0] > per_shape:
+            pick = torch.from_numpy(np.sort(rng.choice(feat.shape[0], per_shape, replace=False))).cuda()
+            feat, off, node = feat[pick], off[pick], node[pick]
+        pts = box.xyz[node.cpu().numpy()] + off.cpu().numpy()
+        if s[0] == "sphere":
+            tgt = np.full(feat.shape[0], 2.0 * h / s[2])
+        else:
+            tgt = wave_closest_hk(pts, *s[1:], h)
+        feats.append(feat)
+        targets.append(tgt)
+        kinds.append(np.full(feat.shape[0], 0 if s[0] == "sphere" else 1, dtype=np.int8))
+        if log and (k + 1) % 50 == 0:
+            log(f"[curvature3d] shapes {k + 1}/{len(shapes)}: {sum(len(t) for t in targets)} samples")
+    return dict(feat=torch.cat(feats), target=np.concatenate(targets), kind=np.concatenate(kinds),
+                reinit_s=t_reinit, samples_s=t_samples)

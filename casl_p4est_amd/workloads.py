@@ -66,7 +66,8 @@ def build_tree(name, lmax=None, log=None):
     if log:
         log(f"[workload] {name}: refine {t1 - t0:.1f}s, nodes {t2 - t1:.1f}s, stencils {t3 - t2:.1f}s; "
             f"{forest.num_nodes} nodes, {forest.num_leaves} leaves")
-    cfg.update(forest=forest, ngbd=ngbd, phi0=phi0, h=float(np.min(forest.dxyz_min())))
+    cfg.update(forest=forest, ngbd=ngbd, phi0=phi0, h=float(np.min(forest.dxyz_min())), shape=shape,
+               build_s=dict(refine=t1 - t0, nodes=t2 - t1, stencils=t3 - t2))
     return cfg
 
 

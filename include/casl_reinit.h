@@ -243,6 +243,37 @@ int casl_curvature_hybrid(const float *feat, const float *pred, int64_t n,
                           float lo, float hi, int residual, float *hk,
                           void *stream);
 
+/* Learned-curvature samples on any grid (2D or 3D), through the stencil
+ * tables: every node with |phi| < band whose 3^dim block of nodes at spacing
+ * h is reachable by face arms of length exactly h that end on a node (x,
+ * then y, then z).  feat[s] = phi/h over the block (x slowest, z fastest),
+ * then h*kappa interpolated inside the block at the projection x - phi*n;
+ * offset[s] =
+ * -phi*n (dim doubles); node[s] = node index.  phi, normals ([dim][N], from
+ * casl_normals) and kappa (from casl_mean_curvature) are FP64 device arrays;
+ * feat is [cap][3^dim + 1] FP32.  *n_samples may exceed max_samples (then
+ * only max_samples rows were written, in no particular order); *n_skipped
+ * counts band nodes without a uniform block.  The 2D features equal those of
+ * kml::Curvature's stencils (my_p4est_curvature_ml.cpp:414-480); the
+ * reference has no 3D counterpart. */
+int casl_curvature_grid_samples(casl_stencil_t *st, const double *phi,
+                                const double *normals, const double *kappa,
+                                double h, double band, float *feat,
+                                double *offset, int64_t *node,
+                                int64_t max_samples, int64_t *n_samples,
+                                int64_t *n_skipped, void *stream);
+/* 3D network inputs: [n][28] samples -> [n][28], negated when ihk > 0, axes
+ * reflected so the centred differences of phi are >= 0 and ordered by
+ * decreasing difference (one row per sample; every symmetry of the cube maps
+ * a stencil to the same row). */
+int casl_curvature_normalize_3d(const float *feat, int64_t n, float *out,
+                                void *stream);
+/* 3D hybrid h*kappa: as casl_curvature_hybrid with one prediction per
+ * sample and ihk = feat[s][27]. */
+int casl_curvature_hybrid_3d(const float *feat, const float *pred, int64_t n,
+                             float lo, float hi, int residual, float *hk,
+                             void *stream);
+
 typedef struct casl_mlp casl_mlp_t;
 /* Fully connected ReLU network with widths dims[0..n_layers] (no ReLU on the
  * output), He-normal initialisation from seed, zero biases. */

@@ -242,3 +242,130 @@ def curvature_samples(phi, h, x0=0.0, y0=0.0):
             ijs.append((i, j))
     return (np.asarray(feats, np.float32).reshape(-1, 10), np.asarray(projs).reshape(-1, 2),
             np.asarray(ijs, np.int64).reshape(-1, 2))
+
+
+# ------------------------------------------------- samples on any grid (2D/3D)
+def _block_offsets(dim):
+    """Offsets of the 3^dim block, x slowest (the feature order)."""
+    r = (-1, 0, 1)
+    if dim == 2:
+        return [(i, j) for i in r for j in r]
+    return [(i, j, k) for i in r for j in r for k in r]
+
+
+def _arm(tables, f, n, h):
+    """Node across face f of node n if the arm is exactly h long and ends on
+    a node: along every transverse axis one of the two transverse distances
+    is zero, and the face corner on that side is the node."""
+    if tables.d[f, n] != h:
+        return None
+    c = 0
+    for t in range(tables.dim - 1):
+        if tables.d_tr[f, 2 * t, n] == 0.0:
+            bit = 0
+        elif tables.d_tr[f, 2 * t + 1, n] == 0.0:
+            bit = 1
+        else:
+            return None
+        c |= bit << t
+    return int(tables.nbr[f, c, n])
+
+
+def grid_block(tables, n, h):
+    """The 3^dim nodes around n reached by arms of length h that end on nodes, walking
+    x, then y, then z (dict offset -> node), or None."""
+    dim = tables.dim
+    blk = {(0,) * dim: n}
+    for s, f in ((-1, 0), (1, 1)):
+        m = _arm(tables, f, n, h)
+        if m is None:
+            return None
+        blk[(s,) + (0,) * (dim - 1)] = m
+    for i in (-1, 0, 1):
+        b = blk[(i,) + (0,) * (dim - 1)]
+        for s, f in ((-1, 2), (1, 3)):
+            m = _arm(tables, f, b, h)
+            if m is None:
+                return None
+            blk[(i, s) + (0,) * (dim - 2)] = m
+    if dim == 3:
+        for i in (-1, 0, 1):
+            for j in (-1, 0, 1):
+                b = blk[(i, j, 0)]
+                for s, f in ((-1, 4), (1, 5)):
+                    m = _arm(tables, f, b, h)
+                    if m is None:
+                        return None
+                    blk[(i, j, s)] = m
+    return blk
+
+
+def grid_samples(tables, phi, normals, kappa, h, band):
+    """Rows for every node with |phi| < band that has a uniform block, in node
+    order: (feat [n][3^dim+1] float32, offset [n][dim], node [n], skipped)."""
+    dim, N = tables.dim, tables.n_nodes
+    offs = _block_offsets(dim)
+    feats, offsets, nodes, skipped = [], [], [], 0
+    for n in np.nonzero(np.abs(phi) < band)[0]:
+        blk = grid_block(tables, int(n), h)
+        if blk is None:
+            skipped += 1
+            continue
+        f0 = phi[n]
+        off = [-f0 * normals[a * N + n] for a in range(dim)]
+        lo, w = [], []
+        for a in range(dim):
+            u = min(max(off[a] / h, -1.0), 1.0)
+            lo.append(-1 if u < 0.0 else 0)
+            w.append(u - lo[-1])
+        k = 0.0
+        for c in range(1 << dim):
+            wt = 1.0
+            o = []
+            for a in range(dim):
+                bit = (c >> a) & 1
+                wt = wt * (w[a] if bit else 1.0 - w[a])
+                o.append(lo[a] + bit)
+            k = k + wt * kappa[blk[tuple(o)]]
+        feats.append([np.float32(phi[blk[o]] / h) for o in offs] + [np.float32(h * k)])
+        offsets.append(off)
+        nodes.append(int(n))
+    q = 3 ** dim + 1
+    return (np.asarray(feats, dtype=np.float32).reshape(-1, q), np.asarray(offsets, dtype=np.float64).reshape(-1, dim),
+            np.asarray(nodes, dtype=np.int64), skipped)
+
+
+def normalize_3d(feat):
+    """[n][28] -> [n][28]: negate when ihk > 0, reflect axes with a negative
+    centred difference, order axes by decreasing difference (ties: lower
+    axis first)."""
+    feat = np.asarray(feat, dtype=np.float32)
+    out = np.empty_like(feat)
+    offs = _block_offsets(3)
+    pos = {o: q for q, o in enumerate(offs)}
+    for s in range(feat.shape[0]):
+        f = feat[s] * (np.float32(-1.0) if feat[s, 27] > 0 else np.float32(1.0))
+        g = [f[pos[(1, 0, 0)]] - f[pos[(-1, 0, 0)]], f[pos[(0, 1, 0)]] - f[pos[(0, -1, 0)]],
+             f[pos[(0, 0, 1)]] - f[pos[(0, 0, -1)]]]
+        g = [np.float32(EPS) if v == 0 else v for v in g]
+        r = [-1 if v < 0 else 1 for v in g]
+        g = [abs(v) for v in g]
+        p = sorted(range(3), key=lambda a: (-g[a], a))
+        for q, (i, j, k) in enumerate(offs):
+            c = [0, 0, 0]
+            c[p[0]], c[p[1]], c[p[2]] = i * r[p[0]], j * r[p[1]], k * r[p[2]]
+            out[s, q] = f[pos[tuple(c)]]
+        out[s, 27] = f[27]
+    return out
+
+
+def hybrid_3d(feat, pred, lo=0.004, hi=0.007, residual=True):
+    feat = np.asarray(feat, dtype=np.float32)
+    pred = np.asarray(pred, dtype=np.float32).reshape(-1)
+    ihk = feat[:, 27]
+    a = np.abs(ihk)
+    p = pred + (-a if residual else np.float32(0))
+    lam = (np.float32(hi) - a) / (np.float32(hi) - np.float32(lo))
+    p = np.where(a <= hi, (np.float32(1) - lam) * p + lam * -a, p)
+    p = np.where(ihk > 0, np.abs(p), -np.abs(p))
+    return np.where(a < lo, ihk, p).astype(np.float32)

@@ -49,3 +49,75 @@ def test_sinusoid_phi0_sign_convention():
     top = np.array([[1.0 + 0.25, 1.0 + 0.1]])
     hk = CV.sinusoid_closest_hk(top, (1.0, 1.0), 0.0, 0.1, 2 * math.pi, 0.0, 1.0)
     assert hk[0] == np.float64(0.1 * (2 * math.pi) ** 2)
+
+
+# ------------------------------------------------------------------ 3D targets
+def _fd_grad_hess(phi, x, eps=1e-5):
+    g = np.empty((x.shape[0], 3))
+    H = np.empty((x.shape[0], 3, 3))
+    for a in range(3):
+        e = np.zeros(3)
+        e[a] = eps
+        g[:, a] = (phi(x + e) - phi(x - e)) / (2 * eps)
+        for b in range(3):
+            f = np.zeros(3)
+            f[b] = eps
+            H[:, a, b] = (phi(x + e + f) - phi(x + e - f) - phi(x - e + f) + phi(x - e - f)) / (4 * eps * eps)
+    return g, H
+
+
+def test_gyroid_gradient_and_hessian_against_finite_differences():
+    phi, gh = CV.gyroid_fields()
+    x = np.random.default_rng(2).uniform(0, 1, (50, 3))
+    g, H = gh(x)
+    gf, Hf = _fd_grad_hess(phi, x)
+    np.testing.assert_allclose(g, gf, rtol=1e-7, atol=1e-6)
+    np.testing.assert_allclose(H, Hf, rtol=1e-4, atol=2e-3 * np.abs(H).max())
+
+
+def test_implicit_curvature_of_a_sphere_is_two_over_r():
+    c, R, h = np.array([0.4, 0.5, 0.6]), 0.23, 2.0 ** -9
+
+    def phi(x):
+        return ((x - c) ** 2).sum(axis=1) - R * R
+
+    def gh(x):
+        H = np.broadcast_to(2 * np.eye(3), (x.shape[0], 3, 3)).copy()
+        return 2 * (x - c), H
+
+    d = np.random.default_rng(4).standard_normal((40, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    pts = c + d * (R + np.random.default_rng(5).uniform(-1.5, 1.5, (40, 1)) * h)
+    np.testing.assert_allclose(CV.implicit_closest_hk(phi, gh, pts, h), 2 * h / R, rtol=1e-12)
+
+
+def test_wave_curvature_agrees_with_the_implicit_formula_on_the_surface():
+    rng = np.random.default_rng(8)
+    h = 2.0 ** -7
+    for _ in range(4):
+        c = 0.5 + rng.uniform(-h, h, 3)
+        Q = CV.random_frame(rng)
+        assert np.allclose(Q.T @ Q, np.eye(3), atol=1e-12)
+        A, om, a, b = rng.uniform(0.01, 0.2), rng.uniform(2 * math.pi, 12 * math.pi), rng.uniform(0, 6), rng.uniform(0, 6)
+        u, v = rng.uniform(-0.4, 0.4, (2, 60))
+        w = A * np.sin(om * u + a) * np.sin(om * v + b)
+        pts = c + np.stack([u, v, w], axis=1) @ Q.T
+        assert np.max(np.abs(CV.wave_phi0(pts, c, Q, A, om, a, b))) < 1e-14
+
+        def phi(x):
+            return CV.wave_phi0(x, c, Q, A, om, a, b)
+
+        def gh(x):
+            p = (x - c) @ Q
+            su, cu = np.sin(om * p[:, 0] + a), np.cos(om * p[:, 0] + a)
+            sv, cv = np.sin(om * p[:, 1] + b), np.cos(om * p[:, 1] + b)
+            gl = np.stack([-A * om * cu * sv, -A * om * su * cv, np.ones(len(p))], axis=1)
+            Hl = np.zeros((len(p), 3, 3))
+            Hl[:, 0, 0] = A * om * om * su * sv
+            Hl[:, 1, 1] = A * om * om * su * sv
+            Hl[:, 0, 1] = Hl[:, 1, 0] = -A * om * om * cu * cv
+            return gl @ Q.T, np.einsum("ia,nab,jb->nij", Q, Hl, Q)
+
+        e = CV.wave_closest_hk(pts, c, Q, A, om, a, b, h)
+        f = CV.implicit_closest_hk(phi, gh, pts, h)
+        np.testing.assert_allclose(e, f, rtol=1e-9, atol=1e-12)

@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 METRIC = "reinit node-iters/s on 1 GPU (mixed vs FP64 at <=1e-3h err); % of peak HBM BW"
 HBM_PEAK = 8.0e12
 SEED = 20261202
+WEIGHTS_3D = os.path.join(os.path.dirname(os.path.abspath(__file__)), "casl_p4est_amd", "weights", "curvature3d.npz")
 
 
 def log(*a):
@@ -277,6 +278,125 @@ def run_curvature(args, torch):
     }
 
 
+def run_curvature3d(args, torch):
+    """Learned h*kappa in 3D: about 1e6 training stencils (3x3x3 phi/h + the
+    interpolated numerical h*kappa) from spheres and wave surfaces on a 65^3
+    uniform box (h = 2^-6), reinitialized on the GPU; 28-128x4-1 BF16 network
+    trained with Adam at batch 16384; mean |h kappa| error of the hybrid
+    estimate against the numerical one on held-out shapes."""
+    from casl_p4est_amd import curvature as CV
+    box = CV.UniformBox(level=6)
+    t0 = time.time()
+    tr = CV.make_dataset_3d(box, 600, 600, seed=SEED, per_shape=1024, log=log)
+    te = CV.make_dataset_3d(box, 100, 100, seed=SEED + 1, per_shape=1024)
+    log(f"[bench] curvature3d data: {len(tr['target'])} train / {len(te['target'])} test samples in "
+        f"{time.time() - t0:.1f}s (reinit {tr['reinit_s']:.1f}s, sampling {tr['samples_s']:.2f}s)")
+    net = CV.CurvatureNet(CV.DIMS_3D, seed=SEED % 1000)
+    hist = CV.train(net, tr["feat"], tr["target"], epochs=args.epochs, batch=16384, lr=2e-3, log=log)
+    if args.save_weights:
+        net.save(args.save_weights, note=f"bench.py --config curvature3d --epochs {args.epochs} (seed {SEED})")
+        log(f"[bench] weights saved to {args.save_weights}")
+    feat, t = te["feat"], te["target"]
+    hk = CV.hybrid_hk(net, feat).cpu().numpy().astype(np.float64)
+    ihk = feat[:, -1].cpu().numpy().astype(np.float64)
+    e_num, e_ml = float(np.abs(ihk - t).mean()), float(np.abs(hk - t).mean())
+    big = np.abs(ihk) >= CV.LO_MIN_HK
+    kinds = {}
+    for k, name in ((0, "spheres"), (1, "waves")):
+        m = te["kind"] == k
+        kinds[name] = dict(numerical=float(np.abs(ihk[m] - t[m]).mean()), learned=float(np.abs(hk[m] - t[m]).mean()),
+                           samples=int(m.sum()))
+    bins = []
+    for lo_b, hi_b in ((0.0, 0.004), (0.004, 0.01), (0.01, 0.03), (0.03, 0.1), (0.1, 0.3), (0.3, 10.0)):
+        m = (np.abs(ihk) >= lo_b) & (np.abs(ihk) < hi_b)
+        if m.any():
+            bins.append(dict(ihk=[lo_b, hi_b], samples=int(m.sum()), numerical=float(np.abs(ihk[m] - t[m]).mean()),
+                             learned=float(np.abs(hk[m] - t[m]).mean())))
+    log(f"[bench] curvature3d: mean |hk err| numerical {e_num:.3e}, learned {e_ml:.3e} (x{e_num / e_ml:.2f}); "
+        f"training {hist['seconds']:.1f}s; by |ihk|: "
+        + ", ".join(f"[{b['ihk'][0]},{b['ihk'][1]}) {b['numerical']:.2e}/{b['learned']:.2e}" for b in bins))
+    return {
+        "metric": "learned 3D h*kappa mean |error| on held-out spheres and wave surfaces (lower is better)",
+        "value": e_ml,
+        "unit": "|h kappa| error",
+        "n_gpus": 1,
+        "steps": hist["steps"],
+        "warmup": 0,
+        "ms_per_step": 1e3 * hist["seconds"] / max(1, hist["steps"]),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16 mfma, fp32 accumulate",
+        "data": "synthetic",
+        "config": {"workload": "h = 2^-6 uniform 65^3 boxes, spheres R log-U[2h,8] and wave surfaces "
+                               "A U[0.02,0.2] k U[pi,6pi], 20 FP64 reinit iterations, 28-128x4-1 ReLU "
+                               "network, Adam, batch 16384", "train_samples": int(len(tr["target"])),
+                   "test_samples": int(len(t)), "epochs": args.epochs},
+        "numerical_error": e_num,
+        "learned_over_numerical": e_ml / e_num,
+        "by_shape": kinds,
+        "by_numerical_hk": bins,
+        "error_where_network_used": {"numerical": float(np.abs(ihk[big] - t[big]).mean()),
+                                     "learned": float(np.abs(hk[big] - t[big]).mean()), "samples": int(big.sum())},
+        "training": {"seconds": hist["seconds"], "rows_per_s": hist["rows"] / hist["seconds"],
+                     "final_loss_unit_variance": hist["loss"][-1]},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
+def curvature_stage(W, phi, torch, weights):
+    """3x3x3-stencil network curvature on every node with |phi| < 1.5 h of a
+    reinitialized tree: numerical normals and mean curvature, samples, network
+    and hybrid, each timed; errors against the level set's exact h*kappa at the
+    projection, on a sample of at most 2e5 nodes."""
+    from casl_p4est_amd import curvature as CV
+    ds = W["ngbd"].device()
+    h = W["h"]
+    net = CV.CurvatureNet.load(weights)
+    torch.cuda.synchronize()
+    a = time.perf_counter()
+    nrm = ds.normals(phi)
+    kap = ds.mean_curvature(nrm)
+    torch.cuda.synchronize()
+    b = time.perf_counter()
+    feat, off, node, skipped = CV.grid_samples(ds, phi, h, 1.5 * h, nrm, kap)
+    torch.cuda.synchronize()
+    c = time.perf_counter()
+    hk = CV.hybrid_hk(net, feat)
+    torch.cuda.synchronize()
+    d = time.perf_counter()
+    n = feat.shape[0]
+    rng = np.random.default_rng(SEED)
+    pick = np.sort(rng.choice(n, min(n, 200000), replace=False))
+    _, xyz = W["forest"].nodes() if n else (None, None)
+    pts = xyz[node.cpu().numpy()[pick]] + off.cpu().numpy()[pick]
+    phi_fn, gh = CV.gyroid_fields() if W.get("name") == "gyroid3d" else (None, None)
+    out = {"band_nodes": int(n + skipped), "samples": int(n), "no_uniform_block": int(skipped),
+           "ms": {"normals_and_mean_curvature": 1e3 * (b - a), "samples": 1e3 * (c - b),
+                  "network_and_hybrid": 1e3 * (d - c)}, "weights": weights}
+    if phi_fn is not None:
+        exact = CV.implicit_closest_hk(phi_fn, gh, pts, h)
+        ihk = feat[:, -1].cpu().numpy()[pick].astype(np.float64)
+        ml = hk.cpu().numpy()[pick].astype(np.float64)
+        used = np.abs(ihk) >= CV.LO_MIN_HK
+        bins = []
+        for lo_b, hi_b in ((0.0, 0.004), (0.004, 0.01), (0.01, 0.03), (0.03, 0.1), (0.1, 0.3), (0.3, 10.0)):
+            m = (np.abs(ihk) >= lo_b) & (np.abs(ihk) < hi_b)
+            if m.any():
+                bins.append(dict(ihk=[lo_b, hi_b], samples=int(m.sum()),
+                                 numerical=float(np.abs(ihk[m] - exact[m]).mean()),
+                                 learned=float(np.abs(ml[m] - exact[m]).mean())))
+        out["by_numerical_hk"] = bins
+        out.update(numerical_error=float(np.abs(ihk - exact).mean()), learned_error=float(np.abs(ml - exact).mean()),
+                   network_used_fraction=float(used.mean()),
+                   error_where_network_used={"numerical": float(np.abs(ihk[used] - exact[used]).mean()) if used.any()
+                                             else None,
+                                             "learned": float(np.abs(ml[used] - exact[used]).mean()) if used.any()
+                                             else None})
+    return out
+
+
 def time_steps(fn, steps, warmup, torch):
     for _ in range(warmup):
         fn()
@@ -347,8 +467,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="spheres3d", choices=["spheres3d", "star2d", "gyroid3d", "batch2d", "curvature2d"])
-    ap.add_argument("--epochs", type=int, default=20, help="training epochs (curvature2d)")
+    ap.add_argument("--config", default="spheres3d",
+                    choices=["spheres3d", "star2d", "gyroid3d", "batch2d", "curvature2d", "curvature3d"])
+    ap.add_argument("--save-weights", default="", help="curvature3d: write the trained network here (.npz)")
+    ap.add_argument("--curvature", default="auto", choices=["auto", "on", "off"],
+                    help="3x3x3 network curvature after the reinitialization (auto: gyroid3d only)")
+    ap.add_argument("--epochs", type=int, default=0,
+                    help="training epochs (curvature2d: 20, curvature3d: 100 when not given)")
     ap.add_argument("--lmax", type=int, default=0, help="reduce max level (quick runs only)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
@@ -368,7 +493,12 @@ def main():
         print(json.dumps(run_batch(args, torch)), flush=True)
         return
     if args.config == "curvature2d":
+        args.epochs = args.epochs or 20
         print(json.dumps(run_curvature(args, torch)), flush=True)
+        return
+    if args.config == "curvature3d":
+        args.epochs = args.epochs or 100
+        print(json.dumps(run_curvature3d(args, torch)), flush=True)
         return
     dev = torch.device("cuda:0")
     W = build_workload(args.config, args.lmax)
@@ -463,22 +593,38 @@ def main():
         g = C.DeviceGrid(dim, W["shape"], W["lmin"], W["lmax"], lip=2.4)
         torch.cuda.synchronize()
         return g
-    g = build_on_device()
-    if g.sizes() != W["forest"].sizes():
-        raise RuntimeError("device-built grid differs from the host-built one")
-    del g
-    t_grid = []
-    for _ in range(max(1, args.steps)):
-        t0 = time.perf_counter()
+    try:
         g = build_on_device()
-        t_grid.append(time.perf_counter() - t0)
+        if g.sizes() != W["forest"].sizes():
+            raise RuntimeError("device-built grid differs from the host-built one")
         del g
-    host_s = sum(W["build_s"].values())
-    grid = {"device_s": min(t_grid), "host_s": host_s, "host_phases_s": W["build_s"],
-            "host_threads": workload_threads(), "speedup": host_s / min(t_grid),
-            "note": "refine_levelset_cf + node numbering + construct_neighbors, tables left in HBM"}
-    log(f"[bench] grid construction: device {grid['device_s']:.3f} s, host {host_s:.2f} s "
-        f"({grid['host_threads']} threads), x{grid['speedup']:.1f}")
+        t_grid = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            g = build_on_device()
+            t_grid.append(time.perf_counter() - t0)
+            del g
+        host_s = sum(W["build_s"].values())
+        grid = {"device_s": min(t_grid), "host_s": host_s, "host_phases_s": W["build_s"],
+                "host_threads": workload_threads(), "speedup": host_s / min(t_grid),
+                "note": "refine_levelset_cf + node numbering + construct_neighbors, tables left in HBM"}
+        log(f"[bench] grid construction: device {grid['device_s']:.3f} s, CPU {host_s:.2f} s "
+            f"({grid['host_threads']} threads), x{grid['speedup']:.1f}")
+    except C.CaslError as e:  # e.g. out of device memory next to the resident workload
+        grid = {"error": str(e)}
+        log(f"[bench] grid construction on the device failed: {e}")
+
+    curv = None
+    want = args.curvature == "on" or (args.curvature == "auto" and args.config == "gyroid3d")
+    if want and dim == 3:
+        if os.path.exists(WEIGHTS_3D):
+            curv = curvature_stage(W, out_mixed, torch, WEIGHTS_3D)
+            log(f"[bench] network curvature on |phi| < 1.5h: {curv['samples']} samples, "
+                + ", ".join(f"{k} {v:.1f} ms" for k, v in curv["ms"].items())
+                + (f"; mean |hk err| numerical {curv['numerical_error']:.3e}, learned {curv['learned_error']:.3e}"
+                   if "learned_error" in curv else ""))
+        else:
+            log(f"[bench] no trained 3D network at {WEIGHTS_3D}: curvature stage skipped")
 
     cpu = None
     if not args.no_cpu:
@@ -509,6 +655,7 @@ def main():
         "effective_hbm_frac": eff,
         "advect_in_normal_direction": advect,
         "grid_construction": grid,
+        "network_curvature": curv,
         "kernels": prof,
         "cpu_baseline": cpu,
     }

@@ -21,7 +21,10 @@
 //             decreasing difference.  Every one of the 48 symmetries of the
 //             cube maps a stencil to the same input: the 3D counterpart of the
 //             2D rotation to the first quadrant plus y = x reflection
-//             (:327-398), with one row per sample instead of two.
+//             (:327-398), with one row per sample instead of two.  The 27
+//             values then go to a basis that BF16 inputs resolve: the centre
+//             value, the half centred differences, and the deviations from
+//             the plane through the centre with those slopes.
 //   3D hybrid  as the 2D one (:483-556): numerical value where |ihk| < lo,
 //             blend towards it up to hi, sign of ihk.
 #include <hip/hip_runtime.h>
@@ -181,7 +184,7 @@ __global__ void __launch_bounds__(BLOCK) k_normalize3(const float *__restrict__ 
       p[1] = y;
     }
   }
-  float *o = out + 28 * s;
+  float v[27];
   for (int i = -1; i <= 1; ++i)
     for (int j = -1; j <= 1; ++j)
       for (int k = -1; k <= 1; ++k) {
@@ -190,8 +193,27 @@ __global__ void __launch_bounds__(BLOCK) k_normalize3(const float *__restrict__ 
         c[p[0]] = i * r[p[0]];
         c[p[1]] = j * r[p[1]];
         c[p[2]] = k * r[p[2]];
-        o[B::q(i, j, k)] = f[B::q(c[0], c[1], c[2])];
+        v[B::q(i, j, k)] = f[B::q(c[0], c[1], c[2])];
       }
+  // the same 27 values in a basis where curvature is not buried under the
+  // linear part (BF16 keeps 8 bits of each input): the centre value, the
+  // three half centred differences g at the -e_a slots, and everywhere else
+  // the deviation from the plane f0 + o.g
+  const float f0 = v[B::q(0, 0, 0)];
+  const float gx = 0.5f * (v[B::q(1, 0, 0)] - v[B::q(-1, 0, 0)]);
+  const float gy = 0.5f * (v[B::q(0, 1, 0)] - v[B::q(0, -1, 0)]);
+  const float gz = 0.5f * (v[B::q(0, 0, 1)] - v[B::q(0, 0, -1)]);
+  float *o = out + 28 * s;
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j)
+      for (int k = -1; k <= 1; ++k) {
+        const int q = B::q(i, j, k);
+        o[q] = v[q] - (f0 + (float(i) * gx + float(j) * gy + float(k) * gz));
+      }
+  o[B::q(0, 0, 0)] = f0;
+  o[B::q(-1, 0, 0)] = gx;
+  o[B::q(0, -1, 0)] = gy;
+  o[B::q(0, 0, -1)] = gz;
   o[27] = f[27];
 }
 

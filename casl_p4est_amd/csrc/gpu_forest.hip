@@ -552,8 +552,7 @@ void build_grid(casl_device_grid *g, int min_lvl, int max_lvl, double lip, doubl
     finest = pass + 1;
   }
   // node numbering
-  const int64_t K = L * CH;
-  if (K >= int64_t(INT32_MAX)) throw std::runtime_error("too many leaf corners for one sort");
+  const int64_t K = L * CH;  // may exceed 2^31: the sort and unique take 64-bit counts
   casl::DevBuf<uint64_t> keys, sorted, nodes;
   casl::DevBuf<int64_t> nsel;
   keys.ensure(size_t(K));
@@ -564,10 +563,10 @@ void build_grid(casl_device_grid *g, int min_lvl, int max_lvl, double lip, doubl
   HIP_CHECK(hipGetLastError());
   const int end_bit = D * GG::MAXLEVEL;
   cub_call(s, [&](void *t, size_t &b) {
-    return hipcub::DeviceRadixSort::SortKeys(t, b, keys.p, sorted.p, int(K), 0, end_bit, s);
+    return hipcub::DeviceRadixSort::SortKeys(t, b, keys.p, sorted.p, K, 0, end_bit, s);
   });
   cub_call(s, [&](void *t, size_t &b) {
-    return hipcub::DeviceSelect::Unique(t, b, sorted.p, nodes.p, nsel.p, int(K), s);
+    return hipcub::DeviceSelect::Unique(t, b, sorted.p, nodes.p, nsel.p, K, s);
   });
   int64_t N = 0;
   HIP_CHECK(hipMemcpyAsync(&N, nsel.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));

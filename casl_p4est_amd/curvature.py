@@ -24,7 +24,7 @@ numerical h*kappa, put in one canonical orientation of the cube
 
     UniformBox              a (2^L)^3 uniform grid on [0, size]^3 as one tree
     grid_samples            the samples of any DeviceStencil
-    make_dataset_3d         spheres and wave surfaces w = A sin(w u + a) sin(w v + b)
+    make_dataset_3d         spheres and wave surfaces w = A sin(k u + a) sin(k v + b)
                             in random frames -> phi0 -> reinitialize_2nd_order
                             (FP64, 20 iterations) -> grid_samples, exact h*kappa
                             (h * div n) at the closest surface point
@@ -259,6 +259,20 @@ class CurvatureNet:
             raise ValueError("parameter sizes do not match the network")
         fp = ctypes.POINTER(ctypes.c_float)
         L.call("casl_mlp_set_params", self._h, w.ctypes.data_as(fp), b.ctypes.data_as(fp))
+
+    def save(self, path, note=""):
+        """Weights, biases, input scaler and widths as an .npz (no pickles)."""
+        w, b = self.params()
+        np.savez(path, dims=np.asarray(self.dims, np.int32), w=w, b=b, mean=self.mean, std=self.std,
+                 note=np.asarray(note))
+
+    @classmethod
+    def load(cls, path):
+        z = np.load(path, allow_pickle=False)
+        net = cls(tuple(int(d) for d in z["dims"]), zero_last_layer=False)
+        net.set_params(z["w"], z["b"])
+        net.set_scaler(z["mean"], z["std"])
+        return net
 
     def grads(self):
         w = np.empty(self.n_weights, np.float32)
@@ -553,8 +567,8 @@ def gyroid_fields(w=8 * math.pi, c=0.3, e=0.5):
 
 def make_dataset_3d(box, n_spheres, n_waves, seed=0, iterations=20, per_shape=4096, log=None):
     """Samples near random spheres (radius log-uniform in [2h, 8], the surface
-    passing within 0.3 of the box centre) and wave surfaces (A U[0.01, 0.2],
-    omega U[2 pi, 12 pi], random phases, frame and offset), each phi0
+    passing within 0.3 of the box centre) and wave surfaces (A U[0.02, 0.2],
+    omega U[pi, 6 pi], random phases, frame and offset), each phi0
     reinitialized by the GPU solver (FP64), at most per_shape samples per
     shape.  Returns dict(feat (device float32 [n][28]), target (numpy exact
     h*kappa), kind (0 sphere, 1 wave), timings)."""
@@ -571,7 +585,7 @@ def make_dataset_3d(box, n_spheres, n_waves, seed=0, iterations=20, per_shape=40
         shapes.append(("sphere", c, R))
     for _ in range(n_waves):
         c = mid + rng.uniform(-h / 2, h / 2, 3)
-        shapes.append(("wave", c, random_frame(rng), rng.uniform(0.01, 0.2), rng.uniform(2 * math.pi, 12 * math.pi),
+        shapes.append(("wave", c, random_frame(rng), rng.uniform(0.02, 0.2), rng.uniform(math.pi, 6 * math.pi),
                        rng.uniform(0, 2 * math.pi), rng.uniform(0, 2 * math.pi)))
     feats, targets, kinds = [], [], []
     t_reinit = t_samples = 0.0

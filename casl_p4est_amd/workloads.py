@@ -50,7 +50,7 @@ def tree_shape(name):
 
 def build_tree(name, lmax=None, log=None):
     """One tree: dict(dim, lmin, lmax, iters, forest, ngbd, phi0, desc, h)."""
-    cfg = dict(TREES[name])
+    cfg = dict(TREES[name], name=name)
     shape = tree_shape(name)
     if lmax:
         cfg["lmax"] = lmax

@@ -338,7 +338,9 @@ def grid_samples(tables, phi, normals, kappa, h, band):
 def normalize_3d(feat):
     """[n][28] -> [n][28]: negate when ihk > 0, reflect axes with a negative
     centred difference, order axes by decreasing difference (ties: lower
-    axis first)."""
+    axis first); then the centre value f0, the half centred differences g at
+    the -e_a slots and, elsewhere, the deviation from the plane f0 + o.g
+    (FP32, in the kernel's operation order)."""
     feat = np.asarray(feat, dtype=np.float32)
     out = np.empty_like(feat)
     offs = _block_offsets(3)
@@ -351,12 +353,41 @@ def normalize_3d(feat):
         r = [-1 if v < 0 else 1 for v in g]
         g = [abs(v) for v in g]
         p = sorted(range(3), key=lambda a: (-g[a], a))
+        v = np.empty(27, np.float32)
         for q, (i, j, k) in enumerate(offs):
             c = [0, 0, 0]
             c[p[0]], c[p[1]], c[p[2]] = i * r[p[0]], j * r[p[1]], k * r[p[2]]
-            out[s, q] = f[pos[tuple(c)]]
+            v[q] = f[pos[tuple(c)]]
+        half = np.float32(0.5)
+        f0 = v[pos[(0, 0, 0)]]
+        gx = half * (v[pos[(1, 0, 0)]] - v[pos[(-1, 0, 0)]])
+        gy = half * (v[pos[(0, 1, 0)]] - v[pos[(0, -1, 0)]])
+        gz = half * (v[pos[(0, 0, 1)]] - v[pos[(0, 0, -1)]])
+        for q, (i, j, k) in enumerate(offs):
+            plane = np.float32(np.float32(np.float32(i) * gx + np.float32(j) * gy) + np.float32(k) * gz)
+            out[s, q] = v[q] - np.float32(f0 + plane)
+        out[s, pos[(0, 0, 0)]] = f0
+        out[s, pos[(-1, 0, 0)]] = gx
+        out[s, pos[(0, -1, 0)]] = gy
+        out[s, pos[(0, 0, -1)]] = gz
         out[s, 27] = f[27]
     return out
+
+
+def denormalize_3d(x):
+    """The canonical stencil values [n][27] back from normalize_3d's basis."""
+    x = np.asarray(x, dtype=np.float64)
+    offs = _block_offsets(3)
+    pos = {o: q for q, o in enumerate(offs)}
+    f0, g = x[:, pos[(0, 0, 0)]], x[:, [pos[(-1, 0, 0)], pos[(0, -1, 0)], pos[(0, 0, -1)]]]
+    v = np.empty((x.shape[0], 27))
+    for q, o in enumerate(offs):
+        v[:, q] = x[:, q] + f0 + g @ np.asarray(o, dtype=np.float64)
+    v[:, pos[(0, 0, 0)]] = f0
+    # the -e_a slot held g_a; its plane deviation equals the +e_a slot's
+    for a, o in enumerate(((-1, 0, 0), (0, -1, 0), (0, 0, -1))):
+        v[:, pos[o]] = f0 - g[:, a] + x[:, pos[tuple(-c for c in o)]]
+    return v
 
 
 def hybrid_3d(feat, pred, lo=0.004, hi=0.007, residual=True):

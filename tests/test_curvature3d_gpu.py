@@ -108,3 +108,17 @@ def test_learned_curvature_3d_beats_numerical(cuda):
         f"all samples {lall:.3e} vs {nall:.3e}")
     assert le < ne
     assert lall <= nall
+
+
+def test_shipped_3d_weights_beat_numerical(cuda):
+    """The network under casl_p4est_amd/weights (trained by bench.py --config
+    curvature3d) on shapes drawn from a seed it was not trained on."""
+    import os
+    path = os.path.join(os.path.dirname(CV.__file__), "weights", "curvature3d.npz")
+    net = CV.CurvatureNet.load(path)
+    assert net.dims == CV.DIMS_3D
+    box = CV.UniformBox(level=6)
+    data = CV.make_dataset_3d(box, 20, 20, seed=99, per_shape=2048)
+    le, ne, lall, nall, used = _errors(net, data)
+    say(f"    shipped weights, network used on {used}: learned {le:.3e} vs numerical {ne:.3e}")
+    assert le < ne and lall < nall

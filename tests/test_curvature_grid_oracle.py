@@ -5,8 +5,10 @@ learned-curvature oracle (oracle/curvature_oracle.py):
     block found from node coordinates, on a uniform and on an adaptive tree;
   * the interpolated h*kappa reproduces a linear kappa field exactly at the
     (clamped) projection point, so a swapped weight or corner fails;
-  * the 3D normalisation maps all 48 symmetries of a stencil to one row, and
-    that row has non-negative, non-increasing centred differences;
+  * the 3D normalisation maps all 48 symmetries of a stencil to one row,
+    whose canonical stencil (recovered from the output basis) holds the
+    input's values and has non-negative, non-increasing centred differences;
+    a linear field leaves no plane deviations;
   * the 3D hybrid at its end points and signs.
 """
 import itertools
@@ -114,19 +116,37 @@ def test_normalize_3d_is_invariant_under_the_cube_symmetries():
                 rows.append(s)
         out = O.normalize_3d(np.asarray(rows))
         assert np.all(out == out[0])
+        v = O.denormalize_3d(out[:1])[0]
         offs = O._block_offsets(3)
         pos = {o: q for q, o in enumerate(offs)}
-        g = [out[0, pos[(1, 0, 0)]] - out[0, pos[(-1, 0, 0)]], out[0, pos[(0, 1, 0)]] - out[0, pos[(0, -1, 0)]],
-             out[0, pos[(0, 0, 1)]] - out[0, pos[(0, 0, -1)]]]
+        g = [v[pos[(1, 0, 0)]] - v[pos[(-1, 0, 0)]], v[pos[(0, 1, 0)]] - v[pos[(0, -1, 0)]],
+             v[pos[(0, 0, 1)]] - v[pos[(0, 0, -1)]]]
         assert g[0] >= g[1] >= g[2] >= 0
         assert out[0, 27] == -abs(base[27])
+        # the canonical stencil holds the input's values, negated when ihk > 0
+        sg = -1.0 if base[27] > 0 else 1.0
+        assert np.allclose(np.sort(v), np.sort(sg * base[:27].astype(np.float64)), rtol=0, atol=1e-5)
 
 
 def test_normalize_3d_keeps_a_canonical_stencil():
-    offs = np.array(O._block_offsets(3), dtype=np.float32)
-    s = np.concatenate([offs @ np.array([3.0, 2.0, 1.0], np.float32) + 0.1 * (offs ** 2).sum(1), [-0.2]])
-    out = O.normalize_3d(s[None].astype(np.float32))
-    assert np.array_equal(out[0], s.astype(np.float32))
+    """A stencil already in canonical orientation keeps its values; the
+    basis holds f0, the half differences and the plane deviations, which
+    vanish for a linear field."""
+    offs = np.array(O._block_offsets(3), dtype=np.float64)
+    quad = 0.25 * (offs ** 2).sum(1)
+    s = np.concatenate([0.5 + offs @ np.array([0.75, 0.5, 0.25]) + quad, [-0.2]]).astype(np.float32)
+    out = O.normalize_3d(s[None])
+    np.testing.assert_allclose(O.denormalize_3d(out)[0], s[:27], atol=1e-6)
+    pos = {tuple(int(v) for v in o): q for q, o in enumerate(offs)}
+    assert out[0, pos[(0, 0, 0)]] == np.float32(0.5)
+    assert [out[0, pos[o]] for o in ((-1, 0, 0), (0, -1, 0), (0, 0, -1))] == [0.75, 0.5, 0.25]
+    for q, o in enumerate(offs):
+        if tuple(o) not in ((0, 0, 0), (-1, 0, 0), (0, -1, 0), (0, 0, -1)):
+            assert out[0, q] == pytest.approx(quad[q], abs=1e-6)
+    lin = np.concatenate([0.5 + offs @ np.array([0.75, 0.5, 0.25]), [-0.2]]).astype(np.float32)
+    out = O.normalize_3d(lin[None])[0]
+    rest = [q for q, o in enumerate(offs) if tuple(o) not in ((0, 0, 0), (-1, 0, 0), (0, -1, 0), (0, 0, -1))]
+    assert np.max(np.abs(out[rest])) <= 1e-6
 
 
 def test_hybrid_3d_end_points():

@@ -121,3 +121,15 @@ def test_wave_curvature_agrees_with_the_implicit_formula_on_the_surface():
         e = CV.wave_closest_hk(pts, c, Q, A, om, a, b, h)
         f = CV.implicit_closest_hk(phi, gh, pts, h)
         np.testing.assert_allclose(e, f, rtol=1e-9, atol=1e-12)
+
+
+def test_shipped_3d_weights_file():
+    import os
+    path = os.path.join(os.path.dirname(CV.__file__), "weights", "curvature3d.npz")
+    z = np.load(path, allow_pickle=False)
+    dims = tuple(int(d) for d in z["dims"])
+    assert dims == CV.DIMS_3D
+    nw = sum(i * o for i, o in zip(dims[:-1], dims[1:]))
+    assert z["w"].shape == (nw,) and z["b"].shape == (sum(dims[1:]),)
+    assert z["mean"].shape == (28,) and np.all(z["std"] > 0)
+    assert np.all(np.isfinite(z["w"])) and np.all(np.isfinite(z["b"]))

@@ -4,6 +4,7 @@ The library is loaded from this package directory.  There is no fallback:
 if the shared library is missing, importing the bindings raises.
 """
 import ctypes
+import weakref
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -124,7 +125,8 @@ SIGNATURES = {
     "casl_curvature_normalize_3d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
                                                    ctypes.c_void_p]),
     "casl_curvature_hybrid_3d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_float,
-                                                ctypes.c_float, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]),
+                                                ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                                                ctypes.c_void_p]),
     "casl_mlp_create": (ctypes.c_int, [c_int32_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "casl_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "casl_mlp_num_params": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p]),
@@ -172,6 +174,19 @@ SIGNATURES = {
 }
 
 _lib = None
+
+
+def own(obj, handle, destroy):
+    """Ties a native handle to obj: the C function `destroy` frees it when obj
+    is collected or, at the latest, when the interpreter exits (before the
+    modules and the HIP runtime are torn down)."""
+    obj._h = handle
+    obj._finalizer = weakref.finalize(obj, _destroy, destroy, handle.value)
+
+
+def _destroy(name, value):
+    if value:
+        getattr(lib(), name)(ctypes.c_void_p(value))
 
 
 def lib():

@@ -24,16 +24,10 @@ class TreeBatch:
             arr[i] = t.struct()
         h = ctypes.c_void_p()
         L.call("casl_batch_create", arr, len(self.trees), ctypes.byref(h))
-        self._h = h
+        L.own(self, h, "casl_batch_destroy")
         self.sizes = np.array([t.n_nodes for t in self.trees], dtype=np.int64)
         self.offsets = np.concatenate([[0], np.cumsum(self.sizes)])
         self.total_nodes = int(self.offsets[-1])
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib().casl_batch_destroy(h)
-            self._h = None
 
     def _check(self, phi):
         torch = _torch()

@@ -26,7 +26,8 @@
 //             value, the half centred differences, and the deviations from
 //             the plane through the centre with those slopes.
 //   3D hybrid  as the 2D one (:483-556): numerical value where |ihk| < lo,
-//             blend towards it up to hi, sign of ihk.
+//             blend towards it up to hi, sign of ihk; a residual network's
+//             corrections at most `deadband` in size are dropped.
 #include <hip/hip_runtime.h>
 
 #include "reinit_device.hpp"
@@ -217,9 +218,12 @@ __global__ void __launch_bounds__(BLOCK) k_normalize3(const float *__restrict__ 
   o[27] = f[27];
 }
 
-// Hybrid h*kappa of 3D samples: one network output per sample.
+// Hybrid h*kappa of 3D samples: one network output per sample.  With a
+// residual network, corrections no larger than `dead` are dropped: on a
+// well-resolved surface the numerical value is more accurate than the
+// network's noise floor.
 __global__ void __launch_bounds__(BLOCK) k_hybrid3(const float *__restrict__ feat, const float *__restrict__ pred,
-                                                   int64_t n, float lo, float hi, int residual,
+                                                   int64_t n, float lo, float hi, int residual, float dead,
                                                    float *__restrict__ hk) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
@@ -229,7 +233,7 @@ __global__ void __launch_bounds__(BLOCK) k_hybrid3(const float *__restrict__ fea
     return;
   }
   float p = pred[s];
-  if (residual) p += -a;
+  if (residual) p = (fabsf(p) > dead ? p : 0.f) + -a;
   if (a <= hi) {
     const float lam = (hi - a) / (hi - lo);
     p = (1.f - lam) * p + lam * -a;
@@ -278,13 +282,13 @@ int casl_curvature_normalize_3d(const float *feat, int64_t n, float *out, void *
 }
 
 int casl_curvature_hybrid_3d(const float *feat, const float *pred, int64_t n, float lo, float hi, int residual,
-                             float *hk, void *stream) {
+                             float deadband, float *hk, void *stream) {
   return guarded([&]() {
-    if (n < 0 || (n > 0 && (!feat || !pred || !hk)) || !(hi > lo) || !(lo >= 0.f))
+    if (n < 0 || (n > 0 && (!feat || !pred || !hk)) || !(hi > lo) || !(lo >= 0.f) || !(deadband >= 0.f))
       throw std::invalid_argument("bad arguments");
     if (n == 0) return;
     hipLaunchKernelGGL(k_hybrid3, dim3(grid_for(n)), dim3(BLOCK), 0, static_cast<hipStream_t>(stream), feat, pred, n,
-                       lo, hi, residual, hk);
+                       lo, hi, residual, deadband, hk);
     HIP_CHECK(hipGetLastError());
   });
 }

@@ -50,6 +50,9 @@ LO_MIN_HK = 0.004
 UP_MIN_HK = 0.007
 DIMS_2D = (10, 128, 128, 128, 128, 1)
 DIMS_3D = (28, 128, 128, 128, 128, 1)
+# 3D corrections no larger than this are dropped (the network's noise floor on
+# well-resolved surfaces, where the numerical value is more accurate)
+DEADBAND_3D = 1e-3
 
 
 def _ptr(t):
@@ -214,7 +217,7 @@ class CurvatureNet:
         d = np.asarray(self.dims, dtype=np.int32)
         h = ctypes.c_void_p()
         L.call("casl_mlp_create", d.ctypes.data_as(L.c_int32_p), len(self.dims) - 1, int(seed), ctypes.byref(h))
-        self._h = h
+        L.own(self, h, "casl_mlp_destroy")
         nw, nb = ctypes.c_int64(), ctypes.c_int64()
         L.call("casl_mlp_num_params", h, ctypes.byref(nw), ctypes.byref(nb))
         self.n_weights, self.n_biases = nw.value, nb.value
@@ -232,12 +235,6 @@ class CurvatureNet:
         w[-self.dims[-2] * self.dims[-1]:] *= np.float32(factor)
         b[-self.dims[-1]:] *= np.float32(factor)
         self.set_params(w, b)
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib().casl_mlp_destroy(h)
-            self._h = None
 
     def set_scaler(self, mean, std):
         self.mean = np.ascontiguousarray(mean, dtype=np.float32)
@@ -401,11 +398,11 @@ def normalize_3d(feat):
     return out
 
 
-def hybrid_3d(feat, pred, lo=LO_MIN_HK, hi=UP_MIN_HK, residual=True):
+def hybrid_3d(feat, pred, lo=LO_MIN_HK, hi=UP_MIN_HK, residual=True, deadband=DEADBAND_3D):
     torch = _torch()
     hk = torch.empty(feat.shape[0], dtype=torch.float32, device=feat.device)
     L.call("casl_curvature_hybrid_3d", _ptr(feat), _ptr(pred), int(feat.shape[0]), float(lo), float(hi),
-           int(bool(residual)), _ptr(hk), _stream_ptr(torch))
+           int(bool(residual)), float(deadband), _ptr(hk), _stream_ptr(torch))
     return hk
 
 

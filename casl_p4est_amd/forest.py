@@ -36,14 +36,8 @@ class Forest:
         h = ctypes.c_void_p()
         L.call("casl_forest_new", dim, self.box_min.ctypes.data_as(L.c_double_p),
                self.box_max.ctypes.data_as(L.c_double_p), ctypes.byref(h))
-        self._h = h
+        L.own(self, h, "casl_forest_destroy")
         self._nodes_built = False
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib().casl_forest_destroy(h)
-            self._h = None
 
     @property
     def handle(self):
@@ -132,15 +126,8 @@ class DeviceGrid:
         L.call("casl_device_grid_build", dim, self.box_min.ctypes.data_as(L.c_double_p),
                self.box_max.ctypes.data_as(L.c_double_p), int(min_lvl), int(max_lvl), float(lip), float(band),
                ctypes.byref(spec), _stream_ptr(torch), ctypes.byref(h))
-        self._h = h
+        L.own(self, h, "casl_device_grid_destroy")
         self._device = None
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            self._device = None
-            L.lib().casl_device_grid_destroy(h)
-            self._h = None
 
     def sizes(self):
         nl, nn, ml = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int32()

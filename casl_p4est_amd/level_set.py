@@ -41,20 +41,15 @@ class DeviceStencil:
         if handle is not None:  # a handle created by the library (casl_device_grid_stencil)
             dim, n = ctypes.c_int32(), ctypes.c_int64()
             L.call("casl_stencil_info", handle, ctypes.byref(dim), ctypes.byref(n))
-            self.dim, self.n_nodes, self._h = dim.value, n.value, handle
+            self.dim, self.n_nodes = dim.value, n.value
+            L.own(self, handle, "casl_stencil_destroy")
             return
         self.dim = tables.dim
         self.n_nodes = tables.n_nodes
         h = ctypes.c_void_p()
         s = tables.struct()
         L.call("casl_stencil_create", ctypes.byref(s), ctypes.byref(h))
-        self._h = h
-
-    def __del__(self):
-        h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            L.lib().casl_stencil_destroy(h)
-            self._h = None
+        L.own(self, h, "casl_stencil_destroy")
 
     @property
     def handle(self):

@@ -269,10 +269,12 @@ int casl_curvature_grid_samples(casl_stencil_t *st, const double *phi,
 int casl_curvature_normalize_3d(const float *feat, int64_t n, float *out,
                                 void *stream);
 /* 3D hybrid h*kappa: as casl_curvature_hybrid with one prediction per
- * sample and ihk = feat[s][27]. */
+ * sample and ihk = feat[s][27]; with residual != 0, predicted corrections
+ * with |pred| <= deadband are dropped (the numerical value is kept where the
+ * network's correction is within its noise floor). */
 int casl_curvature_hybrid_3d(const float *feat, const float *pred, int64_t n,
-                             float lo, float hi, int residual, float *hk,
-                             void *stream);
+                             float lo, float hi, int residual, float deadband,
+                             float *hk, void *stream);
 
 typedef struct casl_mlp casl_mlp_t;
 /* Fully connected ReLU network with widths dims[0..n_layers] (no ReLU on the

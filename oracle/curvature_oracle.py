@@ -390,12 +390,15 @@ def denormalize_3d(x):
     return v
 
 
-def hybrid_3d(feat, pred, lo=0.004, hi=0.007, residual=True):
+def hybrid_3d(feat, pred, lo=0.004, hi=0.007, residual=True, deadband=0.0):
     feat = np.asarray(feat, dtype=np.float32)
     pred = np.asarray(pred, dtype=np.float32).reshape(-1)
     ihk = feat[:, 27]
     a = np.abs(ihk)
-    p = pred + (-a if residual else np.float32(0))
+    if residual:
+        p = np.where(np.abs(pred) > np.float32(deadband), pred, np.float32(0)) + -a
+    else:
+        p = pred
     lam = (np.float32(hi) - a) / (np.float32(hi) - np.float32(lo))
     p = np.where(a <= hi, (np.float32(1) - lam) * p + lam * -a, p)
     p = np.where(ihk > 0, np.abs(p), -np.abs(p))

@@ -78,10 +78,11 @@ def test_normalize_and_hybrid_3d_match_oracle(cuda):
     for k in range(1, len(rows)):
         assert np.array_equal(out[40 * k:40 * (k + 1)], canon)
     pred = rng.standard_normal(len(feat)).astype(np.float32) * 0.01
-    for residual in (True, False):
-        hk = CV.hybrid_3d(torch.from_numpy(feat).to(cuda), torch.from_numpy(pred).to(cuda), residual=residual)
-        np.testing.assert_allclose(hk.cpu().numpy(), O.hybrid_3d(feat, pred, residual=residual), rtol=1e-6,
-                                   atol=1e-9)
+    for residual, dead in ((True, 0.0), (True, 0.005), (False, 0.0)):
+        hk = CV.hybrid_3d(torch.from_numpy(feat).to(cuda), torch.from_numpy(pred).to(cuda), residual=residual,
+                          deadband=dead)
+        np.testing.assert_allclose(hk.cpu().numpy(), O.hybrid_3d(feat, pred, residual=residual, deadband=dead),
+                                   rtol=1e-6, atol=1e-9)
 
 
 def _errors(net, data):

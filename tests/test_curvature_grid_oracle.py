@@ -160,3 +160,8 @@ def test_hybrid_3d_end_points():
     assert hk[4] == pytest.approx(0.05)
     hk = O.hybrid_3d(feat, pred, residual=True)
     assert hk[4] == pytest.approx(abs(0.05 - 0.2))
+    # a correction inside the deadband leaves the numerical value
+    hk = O.hybrid_3d(feat, pred, residual=True, deadband=0.06)
+    assert hk[4] == pytest.approx(0.2)
+    hk = O.hybrid_3d(feat, pred, residual=True, deadband=0.04)
+    assert hk[4] == pytest.approx(abs(0.05 - 0.2))

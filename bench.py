@@ -578,15 +578,21 @@ def main():
     lsr.advect_in_normal_direction(vn, adv, dt=dt_adv)
     torch.cuda.synchronize()
     adv_k = {}
-    for k in ("k_adv_dd", "k_advect"):
+    for k in ("k_adv_dd", "k_advect", "k_tile_advect"):
         ms, cnt = ctypes.c_double(), ctypes.c_int64()
         L.call("casl_profile_query", k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
-        adv_k[k] = dict(avg_ms=ms.value / max(1, cnt.value), launches=cnt.value)
+        if cnt.value:
+            adv_k[k] = dict(avg_ms=ms.value / cnt.value, launches=cnt.value)
     lib.casl_profile_enable(0)
+    other = "two_pass" if args.path == "tiled" else "tiled"
+    W["ngbd"].device().set_path(other)
+    t_other = time_steps(lambda: lsr.advect_in_normal_direction(vn, adv, dt=dt_adv), args.steps, args.warmup, torch)
+    W["ngbd"].device().set_path(args.path)
     advect = {"value": N * args.steps / t_adv, "unit": "node-steps/s (RK2 step, FP64)",
-              "ms_per_step": 1e3 * t_adv / args.steps, "dt": dt_adv, "kernels": adv_k}
+              "ms_per_step": 1e3 * t_adv / args.steps, "dt": dt_adv, "kernels": adv_k, "path": args.path,
+              f"{other}_ms_per_step": 1e3 * t_other / args.steps}
     log(f"[bench] advect_in_normal_direction: {advect['value']:.3e} node-steps/s, "
-        f"{advect['ms_per_step']:.2f} ms per step")
+        f"{advect['ms_per_step']:.2f} ms per step ({other}: {1e3 * t_other / args.steps:.2f} ms)")
 
     # the same grid built on the device (refinement, node numbering, construct_neighbors)
     def build_on_device():

@@ -1,16 +1,21 @@
 // advect_in_normal_direction (src/my_p4est_level_set.cpp:559-726, 785-815)
-// and the time step of its velocity-function form (:744-783), FP64 on the
-// grid-wide tables:
+// and the time step of its velocity-function form (:744-783), FP64:
 //   phi_n = phi;  dd = second_derivatives_central(phi_n)
 //   phi   = stage(vn,    dd, phi_n)            (advect_in_normal_direction_one_iteration)
 //   dd    = second_derivatives_central(phi)
 //   phi   = (phi_n + stage(vn_np1 or vn, dd, phi)) / 2
+// Tiled path (default when the stencil has a tiled encoding): one fused
+// kernel per stage over the Morton tiles, pass A of the stage's input for
+// T u H1 in LDS and the stage for T, phi read once per stage; the first
+// stage writes a scratch array, the second averages into phi in place.
+// Grid-wide path: pass A and the stage as separate kernels.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
 
 #include "reinit_device.hpp"
+#include "tiles_device.hpp"
 
 using namespace casl;
 using namespace casl::dev;
@@ -41,6 +46,84 @@ __global__ void __launch_bounds__(BLOCK) k_advect(Tables<D> t, const double *__r
     out[n] = r;
 }
 
+// One advection stage of a tile: phi of T u H1 u H2 into LDS, pass A for
+// T u H1, then advect_value for T.  out = result, or (RK_AVG) out = (out +
+// result) / 2.  The loads a node needs (its record, slot and vn, and the
+// value being averaged) are issued before the first barrier.
+template <int D, bool RK_AVG>
+__global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu(CASL_TILE_WAVES_F64)))
+    k_tile_advect(TileArgs<double> A, const double *__restrict__ vn, double dt) {
+  constexpr int DMAX = TileLimits<D>::DMAX;
+  constexpr int KT = TileLimits<D>::TMAX / TILE_BLOCK;
+  __shared__ double phis[TileLimits<D>::LMAX];
+  __shared__ double dds[D * DMAX];
+  const int tile = xcd_tile(int(blockIdx.x), int(gridDim.x));
+  const int s = A.tile_start[tile], nT = A.tile_start[tile + 1] - s;
+  const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
+  const int nH1 = A.h1count[tile];
+  const int rb = A.rec_start[tile];
+  TileRec trec[KT];
+  int tslot[KT];
+  double tvn[KT], tprev[KT];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    if (i < nT) {
+      trec[k] = A.rec[rb + i];
+      tslot[k] = A.rec_slot[rb + i];
+    }
+  }
+  for (int j = threadIdx.x; j < nT; j += TILE_BLOCK) phis[j] = A.phin[s + j];
+  for (int j = threadIdx.x; j < nH; j += TILE_BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
+#pragma unroll
+  for (int k = 0; k < KT; ++k)
+    if (int(threadIdx.x) + k * TILE_BLOCK < nT) {
+      tvn[k] = vn[s + tslot[k]];
+      if constexpr (RK_AVG) tprev[k] = A.out[s + tslot[k]];
+    }
+  __syncthreads();
+  auto pass_a = [&](const TileRec &rec, int j) {
+    double f000, fnb[2 * D], dd[D];
+    if (!(rec.geo & GEO_IRREGULAR)) {
+      RegStencil<D, double> st;
+      st.load(rec, A.geom);
+      laplace<D, double>(st, j, phis, f000, fnb, dd);
+    } else {
+      IrrStencil<D, double> st;
+      st.load(rec, A.ext, A.geom);
+      laplace<D, double>(st, j, phis, f000, fnb, dd);
+    }
+#pragma unroll
+    for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
+  };
+#pragma unroll
+  for (int k = 0; k < KT; ++k)
+    if (int(threadIdx.x) + k * TILE_BLOCK < nT) pass_a(trec[k], tslot[k]);
+  for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    if (int(threadIdx.x) + k * TILE_BLOCK >= nT) continue;
+    const TileRec &rec = trec[k];
+    const int j = tslot[k];
+    const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
+    double r;
+    if (!(rec.geo & GEO_IRREGULAR)) {
+      RegStencil<D, double> st;
+      st.load(rec, A.geom);
+      r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
+    } else {
+      IrrStencil<D, double> st;
+      st.load(rec, A.ext, A.geom);
+      r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
+    }
+    if constexpr (RK_AVG)
+      A.out[s + j] = 0.5 * (tprev[k] + r);
+    else
+      A.out[s + j] = r;
+  }
+}
+
 // min over nodes of 0.8 |s_min / vn|, s_min the shortest |arm|; non-negative
 // doubles order like their bit patterns, so an unsigned atomicMin reduces them
 template <int D>
@@ -69,6 +152,23 @@ template <int D>
 void run_advect(casl_stencil *st, double *phi, const double *vn, const double *vnp1, double dt, hipStream_t s) {
   const int64_t N = st->N;
   const Tables<D> t = st->tables<D>();
+  if (st->tiled && st->path == CASL_PATH_TILED) {
+    st->phi0.ensure(N);
+    TileArgs<double> ta = tile_args<double>(st);
+    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK);
+    ta.phin = phi;
+    ta.out = st->phi0.p;
+    prof_begin("k_tile_advect", s);
+    hipLaunchKernelGGL((k_tile_advect<D, false>), g, b, 0, s, ta, vn, dt);
+    prof_end("k_tile_advect", s);
+    ta.phin = st->phi0.p;
+    ta.out = phi;
+    prof_begin("k_tile_advect", s);
+    hipLaunchKernelGGL((k_tile_advect<D, true>), g, b, 0, s, ta, vnp1 ? vnp1 : vn, dt);
+    prof_end("k_tile_advect", s);
+    HIP_CHECK(hipGetLastError());
+    return;
+  }
   st->phi0.ensure(N);
   st->ddn64.ensure(size_t(D) * N);
   double *phin = st->phi0.p, *dd = st->ddn64.p;

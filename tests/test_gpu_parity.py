@@ -260,7 +260,8 @@ def test_first_derivatives_normals_curvature_match_oracle(cuda, name):
 @pytest.mark.parametrize("name", ["star2d", "spheres3d"])
 def test_advect_in_normal_direction_matches_oracle(cuda, name):
     """advect_in_normal_direction (both stage velocities) and its CFL time
-    step through the C ABI against the oracle (FP64)."""
+    step through the C ABI against the oracle (FP64), on the tiled and the
+    grid-wide paths."""
     import torch
     dim, shape, lmin, lmax = PROBLEMS[name]
     say(f"[gpu] advect {name}")
@@ -271,11 +272,20 @@ def test_advect_in_normal_direction_matches_oracle(cuda, name):
     vn = 0.5 + np.sin(4 * xyz[:, 0]) * np.cos(3 * xyz[:, 1]) + 0.1 * rng.standard_normal(tab.n_nodes)
     vn1 = vn * 1.3 - 0.2
     lsr = C.LevelSet(ngbd)
-    p = torch.from_numpy(phi0).to(cuda)
     v = torch.from_numpy(vn).to(cuda)
-    dt = lsr.advect_in_normal_direction(v, p)
-    assert dt == O.advect_cfl_dt(tab, vn)
-    assert rel_err(p.cpu().numpy(), O.advect(tab, phi0, vn, dt)) <= REL
-    p2 = torch.from_numpy(phi0).to(cuda)
-    lsr.advect_in_normal_direction(v, p2, dt=0.5 * dt, vn_np1=torch.from_numpy(vn1).to(cuda))
-    assert rel_err(p2.cpu().numpy(), O.advect(tab, phi0, vn, 0.5 * dt, vn_np1=vn1)) <= REL
+    assert ngbd.device().tile_info()["tiled"]
+    out = {}
+    for path in ("tiled", "two_pass"):
+        ngbd.device().set_path(path)
+        p = torch.from_numpy(phi0).to(cuda)
+        dt = lsr.advect_in_normal_direction(v, p)
+        assert dt == O.advect_cfl_dt(tab, vn)
+        assert rel_err(p.cpu().numpy(), O.advect(tab, phi0, vn, dt)) <= REL
+        p2 = torch.from_numpy(phi0).to(cuda)
+        lsr.advect_in_normal_direction(v, p2, dt=0.5 * dt, vn_np1=torch.from_numpy(vn1).to(cuda))
+        assert rel_err(p2.cpu().numpy(), O.advect(tab, phi0, vn, 0.5 * dt, vn_np1=vn1)) <= REL
+        out[path] = (p.cpu().numpy(), p2.cpu().numpy())
+    ngbd.device().set_path("tiled")
+    # the fused tiled stages and the grid-wide kernels: the same bits
+    assert np.array_equal(out["tiled"][0], out["two_pass"][0])
+    assert np.array_equal(out["tiled"][1], out["two_pass"][1])

@@ -462,6 +462,81 @@ def cpu_baseline(W, budget_s):
                        f"{threads} OpenMP threads, {t1:.1f} s")
 
 
+def extras(args, W, lsr, phi0, out_mixed, dev, torch, C):
+    """advect_in_normal_direction, grid construction on the device and (3D)
+    network curvature on the benchmark's grid."""
+    N, dim = W["ngbd"].num_nodes, W["dim"]
+    # advect_in_normal_direction on the same grid (FP64): one RK2 step per call
+    import ctypes
+    from casl_p4est_amd import _lib as L
+    _, xyz = W["forest"].nodes()
+    vn = torch.from_numpy(1.0 + 0.5 * np.sin(4 * xyz[:, 0])).to(dev)
+    del xyz
+    adv = phi0.clone()
+    dt_adv = lsr.advect_in_normal_direction(vn, adv)
+    t_adv = time_steps(lambda: lsr.advect_in_normal_direction(vn, adv, dt=dt_adv), args.steps, args.warmup, torch)
+    lib = L.lib()
+    lib.casl_profile_enable(1)
+    lib.casl_profile_reset()
+    lsr.advect_in_normal_direction(vn, adv, dt=dt_adv)
+    torch.cuda.synchronize()
+    adv_k = {}
+    for k in ("k_adv_dd", "k_advect", "k_tile_advect"):
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        L.call("casl_profile_query", k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
+        if cnt.value:
+            adv_k[k] = dict(avg_ms=ms.value / cnt.value, launches=cnt.value)
+    lib.casl_profile_enable(0)
+    other = "two_pass" if args.path == "tiled" else "tiled"
+    W["ngbd"].device().set_path(other)
+    t_other = time_steps(lambda: lsr.advect_in_normal_direction(vn, adv, dt=dt_adv), args.steps, args.warmup, torch)
+    W["ngbd"].device().set_path(args.path)
+    advect = {"value": N * args.steps / t_adv, "unit": "node-steps/s (RK2 step, FP64)",
+              "ms_per_step": 1e3 * t_adv / args.steps, "dt": dt_adv, "kernels": adv_k, "path": args.path,
+              f"{other}_ms_per_step": 1e3 * t_other / args.steps}
+    log(f"[bench] advect_in_normal_direction: {advect['value']:.3e} node-steps/s, "
+        f"{advect['ms_per_step']:.2f} ms per step ({other}: {1e3 * t_other / args.steps:.2f} ms)")
+
+    # the same grid built on the device (refinement, node numbering, construct_neighbors)
+    def build_on_device():
+        g = C.DeviceGrid(dim, W["shape"], W["lmin"], W["lmax"], lip=2.4)
+        torch.cuda.synchronize()
+        return g
+    try:
+        g = build_on_device()
+        if g.sizes() != W["forest"].sizes():
+            raise RuntimeError("device-built grid differs from the host-built one")
+        del g
+        t_grid = []
+        for _ in range(max(1, args.steps)):
+            t0 = time.perf_counter()
+            g = build_on_device()
+            t_grid.append(time.perf_counter() - t0)
+            del g
+        host_s = sum(W["build_s"].values())
+        grid = {"device_s": min(t_grid), "host_s": host_s, "host_phases_s": W["build_s"],
+                "host_threads": workload_threads(), "speedup": host_s / min(t_grid),
+                "note": "refine_levelset_cf + node numbering + construct_neighbors, tables left in HBM"}
+        log(f"[bench] grid construction: device {grid['device_s']:.3f} s, CPU {host_s:.2f} s "
+            f"({grid['host_threads']} threads), x{grid['speedup']:.1f}")
+    except C.CaslError as e:  # e.g. out of device memory next to the resident workload
+        grid = {"error": str(e)}
+        log(f"[bench] grid construction on the device failed: {e}")
+
+    curv = None
+    want = args.curvature == "on" or (args.curvature == "auto" and args.config == "gyroid3d")
+    if want and dim == 3:
+        if os.path.exists(WEIGHTS_3D):
+            curv = curvature_stage(W, out_mixed, torch, WEIGHTS_3D)
+            log(f"[bench] network curvature on |phi| < 1.5h: {curv['samples']} samples, "
+                + ", ".join(f"{k} {v:.1f} ms" for k, v in curv["ms"].items())
+                + (f"; mean |hk err| numerical {curv['numerical_error']:.3e}, learned {curv['learned_error']:.3e}"
+                   if "learned_error" in curv else ""))
+        else:
+            log(f"[bench] no trained 3D network at {WEIGHTS_3D}: curvature stage skipped")
+    return advect, grid, curv
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -470,6 +545,8 @@ def main():
     ap.add_argument("--config", default="spheres3d",
                     choices=["spheres3d", "star2d", "gyroid3d", "batch2d", "curvature2d", "curvature3d"])
     ap.add_argument("--save-weights", default="", help="curvature3d: write the trained network here (.npz)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="reinitialization only: skip the advection, grid construction and curvature timings")
     ap.add_argument("--curvature", default="auto", choices=["auto", "on", "off"],
                     help="3x3x3 network curvature after the reinitialization (auto: gyroid3d only)")
     ap.add_argument("--epochs", type=int, default=0,
@@ -563,75 +640,9 @@ def main():
     eff = dict(mixed=rate_mixed * b_alg(dim, 4) / HBM_PEAK, fp64=rate_fp64 * b_alg(dim, 8) / HBM_PEAK,
                b_alg_mixed=b_alg(dim, 4), b_alg_fp64=b_alg(dim, 8))
 
-    # advect_in_normal_direction on the same grid (FP64): one RK2 step per call
-    import ctypes
-    from casl_p4est_amd import _lib as L
-    _, xyz = W["forest"].nodes()
-    vn = torch.from_numpy(1.0 + 0.5 * np.sin(4 * xyz[:, 0])).to(dev)
-    del xyz
-    adv = phi0.clone()
-    dt_adv = lsr.advect_in_normal_direction(vn, adv)
-    t_adv = time_steps(lambda: lsr.advect_in_normal_direction(vn, adv, dt=dt_adv), args.steps, args.warmup, torch)
-    lib = L.lib()
-    lib.casl_profile_enable(1)
-    lib.casl_profile_reset()
-    lsr.advect_in_normal_direction(vn, adv, dt=dt_adv)
-    torch.cuda.synchronize()
-    adv_k = {}
-    for k in ("k_adv_dd", "k_advect", "k_tile_advect"):
-        ms, cnt = ctypes.c_double(), ctypes.c_int64()
-        L.call("casl_profile_query", k.encode(), ctypes.byref(ms), ctypes.byref(cnt))
-        if cnt.value:
-            adv_k[k] = dict(avg_ms=ms.value / cnt.value, launches=cnt.value)
-    lib.casl_profile_enable(0)
-    other = "two_pass" if args.path == "tiled" else "tiled"
-    W["ngbd"].device().set_path(other)
-    t_other = time_steps(lambda: lsr.advect_in_normal_direction(vn, adv, dt=dt_adv), args.steps, args.warmup, torch)
-    W["ngbd"].device().set_path(args.path)
-    advect = {"value": N * args.steps / t_adv, "unit": "node-steps/s (RK2 step, FP64)",
-              "ms_per_step": 1e3 * t_adv / args.steps, "dt": dt_adv, "kernels": adv_k, "path": args.path,
-              f"{other}_ms_per_step": 1e3 * t_other / args.steps}
-    log(f"[bench] advect_in_normal_direction: {advect['value']:.3e} node-steps/s, "
-        f"{advect['ms_per_step']:.2f} ms per step ({other}: {1e3 * t_other / args.steps:.2f} ms)")
-
-    # the same grid built on the device (refinement, node numbering, construct_neighbors)
-    def build_on_device():
-        g = C.DeviceGrid(dim, W["shape"], W["lmin"], W["lmax"], lip=2.4)
-        torch.cuda.synchronize()
-        return g
-    try:
-        g = build_on_device()
-        if g.sizes() != W["forest"].sizes():
-            raise RuntimeError("device-built grid differs from the host-built one")
-        del g
-        t_grid = []
-        for _ in range(max(1, args.steps)):
-            t0 = time.perf_counter()
-            g = build_on_device()
-            t_grid.append(time.perf_counter() - t0)
-            del g
-        host_s = sum(W["build_s"].values())
-        grid = {"device_s": min(t_grid), "host_s": host_s, "host_phases_s": W["build_s"],
-                "host_threads": workload_threads(), "speedup": host_s / min(t_grid),
-                "note": "refine_levelset_cf + node numbering + construct_neighbors, tables left in HBM"}
-        log(f"[bench] grid construction: device {grid['device_s']:.3f} s, CPU {host_s:.2f} s "
-            f"({grid['host_threads']} threads), x{grid['speedup']:.1f}")
-    except C.CaslError as e:  # e.g. out of device memory next to the resident workload
-        grid = {"error": str(e)}
-        log(f"[bench] grid construction on the device failed: {e}")
-
-    curv = None
-    want = args.curvature == "on" or (args.curvature == "auto" and args.config == "gyroid3d")
-    if want and dim == 3:
-        if os.path.exists(WEIGHTS_3D):
-            curv = curvature_stage(W, out_mixed, torch, WEIGHTS_3D)
-            log(f"[bench] network curvature on |phi| < 1.5h: {curv['samples']} samples, "
-                + ", ".join(f"{k} {v:.1f} ms" for k, v in curv["ms"].items())
-                + (f"; mean |hk err| numerical {curv['numerical_error']:.3e}, learned {curv['learned_error']:.3e}"
-                   if "learned_error" in curv else ""))
-        else:
-            log(f"[bench] no trained 3D network at {WEIGHTS_3D}: curvature stage skipped")
-
+    advect = grid = curv = None
+    if not args.no_extras:
+        advect, grid, curv = extras(args, W, lsr, phi0, out_mixed, dev, torch, C)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(W, args.cpu_budget)

@@ -84,16 +84,25 @@ struct IrrStencil : DecodedDist<D, R> {
   static constexpr bool REGULAR = false;
   static constexpr int F = 2 * D, C = 1 << (D - 1);
   static constexpr int W = (D == 2 ? 3 : 6);  // uint16 words per face extension
-  const uint16_t *__restrict__ e;
+  static constexpr int NW = F * W / 2;        // 32-bit words per record (8-byte aligned)
+  uint32_t e[NW];                             // the record, loaded once with 8-byte loads
   R unit;
-  __device__ __forceinline__ int32_t idx(int f, int c, int64_t) const { return e[f * W + c]; }
+  __device__ __forceinline__ uint32_t word(int i) const { return (e[i >> 1] >> (16 * (i & 1))) & 0xffffu; }
+  __device__ __forceinline__ int32_t idx(int f, int c, int64_t) const { return int32_t(word(f * W + c)); }
   __device__ __forceinline__ R dt(int f, int k, int64_t) const {
-    const R o = R(e[f * W + C + (k >> 1)]) * unit;  // m0 (k = 0, 1) or 0m (k = 2, 3)
+    const R o = R(word(f * W + C + (k >> 1))) * unit;  // m0 (k = 0, 1) or 0m (k = 2, 3)
     return (k & 1) ? this->q[f] - o : o;
   }
   __device__ __forceinline__ void load(const TileRec &r, const uint16_t *__restrict__ ext, const TileGeom<R> &g) {
+    static_assert(NW % 2 == 0, "ext records are whole 8-byte words");
     const uint32_t id = uint32_t(r.idx[0]) | (uint32_t(r.idx[1]) << 16);
-    e = ext + size_t(id) * (F * W);
+    const uint2 *src = reinterpret_cast<const uint2 *>(ext + size_t(id) * (F * W));
+#pragma unroll
+    for (int i = 0; i < NW / 2; ++i) {
+      const uint2 v = src[i];
+      e[2 * i] = v.x;
+      e[2 * i + 1] = v.y;
+    }
     unit = g.unit[0];
     this->decode(r.geo, g);
   }
@@ -174,13 +183,57 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
       tsf[k] = uint32_t(A.rec_slot[rb + i]) | (uint32_t(A.flags[s + i]) << 16);
     }
   }
-  for (int j = threadIdx.x; j < nT; j += BLOCK) phis[j] = A.phin[s + j];
-  for (int j = threadIdx.x; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
+  // the first K1 rows of H1 records (pass A), also loaded before the barrier
+  constexpr int K1 = 2;
+  TileRec hrec[K1];
+  int hslot[K1];
+  if constexpr (SECOND_ORDER) {
+#pragma unroll
+    for (int k = 0; k < K1; ++k) {
+      const int i = nT + int(threadIdx.x) + k * BLOCK;
+      if (i < nT + nH1) {
+        hrec[k] = A.rec[rb + i];
+        hslot[k] = A.rec_slot[rb + i];
+      }
+    }
+  }
+  // the phi gather, unrolled so that a thread's loads are in flight together:
+  // the halo ids of the first KH rows, then every phi value, then the LDS
+  // stores (the rare tail beyond KH rows of halo takes the plain loop)
+  constexpr int KH = (TileLimits<D>::LMAX - TileLimits<D>::TMAX) / BLOCK / 2;
+  int hid[KH];
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    const int j = int(threadIdx.x) + k * BLOCK;
+    if (j < nH) hid[k] = A.halo[hb + j];
+  }
+  R tv[KT], hv[KH];
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int j = int(threadIdx.x) + k * BLOCK;
+    if (j < nT) tv[k] = A.phin[s + j];
+  }
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    const int j = int(threadIdx.x) + k * BLOCK;
+    if (j < nH) hv[k] = A.phin[hid[k]];
+  }
   if constexpr (RK_AVG) {
 #pragma unroll
     for (int k = 0; k < KT; ++k)
       if (int(threadIdx.x) + k * BLOCK < nT) tprev[k] = A.out[s + int(tsf[k] & 0xffffu)];
   }
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int j = int(threadIdx.x) + k * BLOCK;
+    if (j < nT) phis[j] = tv[k];
+  }
+#pragma unroll
+  for (int k = 0; k < KH; ++k) {
+    const int j = int(threadIdx.x) + k * BLOCK;
+    if (j < nH) phis[nT + j] = hv[k];
+  }
+  for (int j = int(threadIdx.x) + KH * BLOCK; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();
   if constexpr (SECOND_ORDER) {
     auto pass_a = [&](const TileRec &rec, int j) {
@@ -200,7 +253,10 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
 #pragma unroll
     for (int k = 0; k < KT; ++k)
       if (int(threadIdx.x) + k * BLOCK < nT) pass_a(trec[k], int(tsf[k] & 0xffffu));
-    for (int i = nT + int(threadIdx.x); i < nT + nH1; i += BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
+#pragma unroll
+    for (int k = 0; k < K1; ++k)
+      if (nT + int(threadIdx.x) + k * BLOCK < nT + nH1) pass_a(hrec[k], hslot[k]);
+    for (int i = nT + int(threadIdx.x) + K1 * BLOCK; i < nT + nH1; i += BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
     __syncthreads();
   }
 #pragma unroll

@@ -107,6 +107,20 @@ void require_device() {
 }  // namespace
 
 namespace casl {
+namespace {
+// The word of a float, or the high word of a double, that holds its exponent.
+uint32_t exp_word(float x) {
+  uint32_t b;
+  std::memcpy(&b, &x, sizeof(b));
+  return b;
+}
+uint32_t exp_word(double x) {
+  uint64_t b;
+  std::memcpy(&b, &x, sizeof(b));
+  return uint32_t(b >> 32);
+}
+}  // namespace
+
 // The tile arrays and geometry of a stencil's tiled encoding, in arithmetic R
 // (per-call fields -- flags, s_fix, phi_0, phi in/out -- are the caller's).
 template <class R>
@@ -127,7 +141,11 @@ dev::TileArgs<R> tile_args(const casl_stencil *st) {
     int e = 0;
     const double m = std::frexp(st->geom_base[a], &e);
     ta.geom.inv_base[a] = R(1.0 / st->geom_base[a]);
-    if (m != 0.5) ta.geom.pow2 = 0;
+    // exponent arithmetic needs a normal base whose arm cells (down to 2^-15
+    // of it) and their inverses stay normal: true for any real tree
+    if (m != 0.5 || e < -90 || e > 90) ta.geom.pow2 = 0;
+    ta.geom.base_bits[a] = exp_word(ta.geom.base[a]);
+    ta.geom.inv_bits[a] = exp_word(ta.geom.inv_base[a]);
   }
   ta.N = st->N;
   return ta;

@@ -380,8 +380,14 @@ def train(net, feat, target, epochs=10, batch=16384, lr=1e-3, lr_final=1e-5, see
     return dict(loss=hist, seconds=seconds, steps=step, rows=step * batch, target_scale=sigma)
 
 
-def hybrid_hk(net, feat, residual=True):
-    """Learned h*kappa of samples feat ([n][10] or [n][28], device)."""
+def hybrid_hk(net, feat, residual=True, chunk=1 << 22):
+    """Learned h*kappa of samples feat ([n][10] or [n][28], device), in
+    chunks of at most `chunk` samples (the network's activations are sized by
+    the rows of one call)."""
+    torch = _torch()
+    if feat.shape[0] > chunk:
+        return torch.cat([hybrid_hk(net, feat[i:i + chunk], residual, chunk)
+                          for i in range(0, feat.shape[0], chunk)])
     if feat.shape[1] == 28:
         return hybrid_3d(feat, net.forward(normalize_3d(feat)), residual=residual)
     x = normalize(feat)

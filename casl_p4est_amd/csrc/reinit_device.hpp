@@ -227,19 +227,22 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
     // MAX(s, zero_distance_threshold) of the non-crossing arms only changes
     // negative wall arms, which the Neumann mirror below overwrites.
   }
-  // Neumann boundary condition on the walls (:216-225)
+  // Neumann boundary condition on the walls (:216-225); nodes off the walls
+  // (all but a thin layer) skip the selects
+  if (w) {
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    const int fm = 2 * a, fp = 2 * a + 1;
-    if (w & (1u << fm)) {
-      s[fm] = s[fp];
-      pn_f[fm] = pn_f[fp];
-      if constexpr (SECOND_ORDER) pn_dd_000[a] = pn_dd_f[fm] = pn_dd_f[fp] = R(0.0);
-    }
-    if (w & (1u << fp)) {
-      s[fp] = s[fm];
-      pn_f[fp] = pn_f[fm];
-      if constexpr (SECOND_ORDER) pn_dd_000[a] = pn_dd_f[fm] = pn_dd_f[fp] = R(0.0);
+    for (int a = 0; a < D; ++a) {
+      const int fm = 2 * a, fp = 2 * a + 1;
+      if (w & (1u << fm)) {
+        s[fm] = s[fp];
+        pn_f[fm] = pn_f[fp];
+        if constexpr (SECOND_ORDER) pn_dd_000[a] = pn_dd_f[fm] = pn_dd_f[fp] = R(0.0);
+      }
+      if (w & (1u << fp)) {
+        s[fp] = s[fm];
+        pn_f[fp] = pn_f[fm];
+        if constexpr (SECOND_ORDER) pn_dd_000[a] = pn_dd_f[fm] = pn_dd_f[fp] = R(0.0);
+      }
     }
   }
   // one-sided differences with ENO/minmod correction (:230-248)
@@ -291,10 +294,12 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
     dt *= R(1.0 / D);
   else
     dt /= R(D);
+  // upwinding: D+ is dropped where sgn*D+ > 0 and D- where sgn*D- < 0; only
+  // the squares enter, and (sgn*x)^2 == x^2 exactly, so clamp sgn*x instead
 #pragma unroll
   for (int a = 0; a < D; ++a) {
-    if (sgn * Dp[a] > R(0.0)) Dp[a] = R(0.0);
-    if (sgn * Dm[a] < R(0.0)) Dm[a] = R(0.0);
+    Dp[a] = fmin(sgn * Dp[a], R(0.0));
+    Dm[a] = fmax(sgn * Dm[a], R(0.0));
   }
   R sum = max2(Dp[0] * Dp[0], Dm[0] * Dm[0]) + max2(Dp[1] * Dp[1], Dm[1] * Dm[1]);
   if constexpr (D == 3) sum = sum + max2(Dp[2] * Dp[2], Dm[2] * Dm[2]);

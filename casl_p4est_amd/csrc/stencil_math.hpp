@@ -42,6 +42,10 @@ __device__ __forceinline__ R minmod(R a, R b) {
   if (fabs(a) < fabs(b)) return a;
   return b;
 }
+// FP32 (the mixed path only): the median of a, b and 0, one instruction.  It
+// is MINMOD's value except where a*b underflows to zero (|a*b| < 1e-38),
+// there it keeps the smaller of two values below 1e-19 instead of 0.
+__device__ __forceinline__ float minmod(float a, float b) { return __builtin_amdgcn_fmed3f(a, b, 0.f); }
 template <class R>
 __host__ __device__ __forceinline__ R min2(R a, R b) { return a < b ? a : b; }
 template <class R>

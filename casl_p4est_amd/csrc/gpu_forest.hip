@@ -32,7 +32,11 @@ using namespace casl;
 namespace {
 
 constexpr int BLOCK = 256;
-inline unsigned grid_for(int64_t n) { return unsigned((n + BLOCK - 1) / BLOCK); }
+// One thread per item; a launch holds fewer than 2^32 work-items.
+inline unsigned grid_for(int64_t n) {
+  if (n >= (int64_t(1) << 32) - BLOCK) throw std::runtime_error("grid too large for one launch");
+  return unsigned((n + BLOCK - 1) / BLOCK);
+}
 
 struct DQuad {
   int32_t c[3];

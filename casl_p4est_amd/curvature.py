@@ -423,7 +423,7 @@ def grid_samples(ds, phi, h, band, normals=None, kappa=None):
     if kappa is None:
         kappa = ds.mean_curvature(normals)
     nf = 3 ** ds.dim + 1
-    cap = max(1024, ds.n_nodes // 8)
+    cap = max(1024, int((phi.abs() < band).sum()))  # every sample is a band node
     while True:
         feat = torch.empty((cap, nf), dtype=torch.float32, device=phi.device)
         off = torch.empty((cap, ds.dim), dtype=torch.float64, device=phi.device)

@@ -260,8 +260,8 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
       if (cross) {
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-          if (cross & (2u << (2 * a))) Dp[a] = (pn_f[2 * a + 1] - pn_000) / s[2 * a + 1];
-          if (cross & (1u << (2 * a))) Dm[a] = (pn_000 - pn_f[2 * a]) / s[2 * a];
+          if (cross & (2u << (2 * a))) Dp[a] = qdiv(pn_f[2 * a + 1] - pn_000, s[2 * a + 1]);
+          if (cross & (1u << (2 * a))) Dm[a] = qdiv(pn_000 - pn_f[2 * a], s[2 * a]);
         }
       }
       divided = true;
@@ -270,8 +270,8 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
   if (!divided) {
 #pragma unroll
     for (int a = 0; a < D; ++a) {
-      Dp[a] = (pn_f[2 * a + 1] - pn_000) / s[2 * a + 1];
-      Dm[a] = (pn_000 - pn_f[2 * a]) / s[2 * a];
+      Dp[a] = qdiv(pn_f[2 * a + 1] - pn_000, s[2 * a + 1]);
+      Dm[a] = qdiv(pn_000 - pn_f[2 * a], s[2 * a]);
     }
   }
   if constexpr (SECOND_ORDER) {

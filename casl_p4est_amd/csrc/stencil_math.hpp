@@ -78,6 +78,19 @@ __device__ __forceinline__ double interface_location_2nd(double a, double b, dou
   return x + 0.5 * (a + b);
 }
 
+// x / y.  FP64 divides (the path that reproduces the reference bit for bit);
+// FP32 -- the mixed path, judged against FP64 at 1e-3 h -- multiplies by the
+// hardware reciprocal (about 1 ulp), two instructions instead of a range-safe
+// division sequence.
+template <class R>
+__device__ __forceinline__ R qdiv(R x, R y) {
+  return x / y;
+}
+template <>
+__device__ __forceinline__ float qdiv<float>(float x, float y) {
+  return x * __builtin_amdgcn_rcpf(y);
+}
+
 // Exact-inverse division.  A stencil accessor with TB::FASTDIV and t.fast
 // has power-of-two face distances d_f = 1 / iq[f] (cubic dyadic cells of a
 // power-of-two tree, no wall arm).  Dividing by a power of two and
@@ -98,7 +111,7 @@ __device__ __forceinline__ R div_d(const TB &t, R x, int f, int64_t n) {
   if constexpr (irregular_fastdiv<R, TB>()) {
     if (t.fast) return x * t.iq[f];
   }
-  return x / R(t.d(f, n));
+  return qdiv(x, R(t.d(f, n)));
 }
 // x / (d_{2a+1} + d_{2a}): a power of two when both arms are equal
 template <class R, class TB>
@@ -106,7 +119,7 @@ __device__ __forceinline__ R div_dsum(const TB &t, R x, int a, int64_t n) {
   if constexpr (irregular_fastdiv<R, TB>()) {
     if (t.fast && t.dist[2 * a + 1] == t.dist[2 * a]) return x * (R(0.5) * t.iq[2 * a + 1]);
   }
-  return x / (R(t.d(2 * a + 1, n)) + R(t.d(2 * a, n)));
+  return qdiv(x, R(t.d(2 * a + 1, n)) + R(t.d(2 * a, n)));
 }
 
 // get_linear_interpolation_weights + interpolate_linearly,
@@ -135,7 +148,7 @@ __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G 
     if constexpr (irregular_fastdiv<R, TB>()) {
       if (t.fast) return num * t.iq[f];
     }
-    return num / normalization;
+    return qdiv(num, normalization);
   } else {
     const R d_0m = R(t.dt(f, 2, n)), d_0p = R(t.dt(f, 3, n));
     const R w_mm = d_p0 * d_0p;
@@ -149,14 +162,14 @@ __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G 
     if constexpr (irregular_fastdiv<R, TB>()) {
       if (t.fast) return num * (t.iq[f] * t.iq[f]);
     }
-    return num / normalization;
+    return qdiv(num, normalization);
   }
 }
 
 // central_second_derivative, my_p4est_quad_neighbor_nodes_of_node.h:214-220
 template <class R>
 __device__ __forceinline__ R central_second_derivative(R fp, R f0, R fm, R dp, R dm) {
-  return ((fp - f0) / dp + (fm - f0) / dm) * R(2.) / (dp + dm);
+  return qdiv((qdiv(fp - f0, dp) + qdiv(fm - f0, dm)) * R(2.), dp + dm);
 }
 
 // central_second_derivative along axis a of a stencil.  When the accessor
@@ -170,7 +183,7 @@ __device__ __forceinline__ R second_difference(const TB &t, int a, R fp, R f0, R
   if constexpr (TB::FASTDIV) {
     if (t.fast) {
       const R x = ((fp - f0) * t.iq[fpp] + (fm - f0) * t.iq[fmm]) * R(2.);
-      return t.dist[fpp] == t.dist[fmm] ? x * (R(0.5) * t.iq[fpp]) : x / (t.dist[fpp] + t.dist[fmm]);
+      return t.dist[fpp] == t.dist[fmm] ? x * (R(0.5) * t.iq[fpp]) : qdiv(x, t.dist[fpp] + t.dist[fmm]);
     }
   }
   return central_second_derivative(fp, f0, fm, R(t.d(fpp, n)), R(t.d(fmm, n)));
@@ -193,7 +206,7 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
     } else if (check_if_zero(m21)) {
       Dxx -= m12 * Dyy;
     } else {
-      const R idet = R(1.0) / (R(1.0) - m12 * m21);
+      const R idet = qdiv(R(1.0), R(1.0) - m12 * m21);
       const R nDxx = Dxx, nDyy = Dyy;
       Dxx = nDxx * idet - nDyy * (m12 * idet);
       Dyy = nDyy * idet - nDxx * (m21 * idet);
@@ -219,7 +232,7 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
         Dyy -= m23 * Dzz;
         if (z21) Dyy -= m21 * Dxx;
       } else {
-        const R idet = R(1.0) / (R(1.0) - m23 * m32);
+        const R idet = qdiv(R(1.0), R(1.0) - m23 * m32);
         const bool i21 = check_if_zero(m21), i31 = check_if_zero(m31);
         const R nDyy = Dyy, nDzz = Dzz;
         Dyy = ((i21 ? nDyy : (nDyy - m21 * Dxx)) - m23 * (i31 ? nDzz : (nDzz - m31 * Dxx))) * idet;
@@ -237,7 +250,7 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
         Dxx -= m13 * Dzz;
         if (z12) Dxx -= m12 * Dyy;
       } else {
-        const R idet = R(1.0) / (R(1.0) - m13 * m31);
+        const R idet = qdiv(R(1.0), R(1.0) - m13 * m31);
         const bool i12 = check_if_zero(m12), i32 = check_if_zero(m32);
         const R nDxx = Dxx, nDzz = Dzz;
         Dxx = ((i12 ? nDxx : (nDxx - m12 * Dyy)) - m13 * (i32 ? nDzz : (nDzz - m32 * Dyy))) * idet;
@@ -255,7 +268,7 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
         Dxx -= m12 * Dyy;
         if (z13) Dxx -= m13 * Dzz;
       } else {
-        const R idet = R(1.0) / (R(1.0) - m12 * m21);
+        const R idet = qdiv(R(1.0), R(1.0) - m12 * m21);
         const bool i13 = check_if_zero(m13), i23 = check_if_zero(m23);
         const R nDxx = Dxx, nDyy = Dyy;
         Dxx = ((i13 ? nDxx : (nDxx - m13 * Dzz)) - m12 * (i23 ? nDyy : (nDyy - m23 * Dzz))) * idet;
@@ -263,7 +276,7 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
       }
     } else {
       const R idet =
-          R(1.0) / (R(1.0) + m12 * m23 * m31 + m13 * m21 * m32 - m12 * m21 - m13 * m31 - m23 * m32);
+          qdiv(R(1.0), R(1.0) + m12 * m23 * m31 + m13 * m21 * m32 - m12 * m21 - m13 * m31 - m23 * m32);
       const R nDxx = Dxx, nDyy = Dyy, nDzz = Dzz;
       Dxx = (nDxx * (idet * (R(1.0) - m23 * m32)) + nDyy * (idet * (m32 * m13 - m12)) +
              nDzz * (idet * (m12 * m23 - m13)));

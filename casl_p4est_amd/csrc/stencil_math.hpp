@@ -91,6 +91,17 @@ __device__ __forceinline__ float qdiv<float>(float x, float y) {
   return x * __builtin_amdgcn_rcpf(y);
 }
 
+// sqrt: correctly rounded in FP64; the hardware square root in FP32 (the
+// mixed path), without the denormal rescaling around it.
+template <class R>
+__device__ __forceinline__ R qsqrt(R x) {
+  return sqrt(x);
+}
+template <>
+__device__ __forceinline__ float qsqrt<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+
 // Exact-inverse division.  A stencil accessor with TB::FASTDIV and t.fast
 // has power-of-two face distances d_f = 1 / iq[f] (cubic dyadic cells of a
 // power-of-two tree, no wall arm).  Dividing by a power of two and

@@ -608,6 +608,15 @@ def main():
     out_mixed = phi.clone()
     t_fp64 = time_steps(step("fp64"), args.steps, args.warmup, torch)
     out_fp64 = phi.clone()
+    # the same calls replayed from a recorded hipGraph (casl_reinit_graph)
+    lsr.graph = True
+    t_mixed_g = time_steps(step("mixed"), args.steps, args.warmup, torch)
+    graph_ok = bool(torch.equal(phi, out_mixed))
+    t_fp64_g = time_steps(step("fp64"), args.steps, args.warmup, torch)
+    graph_ok = graph_ok and bool(torch.equal(phi, out_fp64))
+    lsr.graph = False
+    if not graph_ok:
+        raise RuntimeError("graph replay differs from the eager calls")
     rate_mixed = N * iters * args.steps / t_mixed
     rate_fp64 = N * iters * args.steps / t_fp64
     err = W["ngbd"].device().band_linf(out_mixed, out_fp64, out_fp64, 3 * h)
@@ -621,7 +630,8 @@ def main():
     t_host = time.perf_counter() - t0
     rate_host = N * iters / t_host
     log(f"[bench] mixed {rate_mixed:.3e} node-iters/s, fp64 {rate_fp64:.3e}, x{rate_mixed / rate_fp64:.2f}, "
-        f"Linf band err {err:.2e} = {err / h:.2e} h")
+        f"Linf band err {err:.2e} = {err / h:.2e} h; graph replay mixed "
+        f"{N * iters * args.steps / t_mixed_g:.3e}, fp64 {N * iters * args.steps / t_fp64_g:.3e}")
 
     prof = {}
     for prec, w in (("mixed", 4), ("fp64", 8)):
@@ -675,6 +685,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": W["desc"], "nodes": N, "iterations": iters, "h_min": h, "path": args.path},
         "fp64": {"value": rate_fp64, "unit": "node-iters/s", "ms_per_step": 1e3 * t_fp64 / args.steps},
+        "graph_replay": {"mixed": N * iters * args.steps / t_mixed_g, "fp64": N * iters * args.steps / t_fp64_g,
+                         "unit": "node-iters/s", "note": "casl_reinit_graph: the same calls replayed from a "
+                                                         "recorded hipGraph, results identical"},
         "mixed_over_fp64": rate_mixed / rate_fp64,
         "pcie_inclusive": {"value": rate_host, "unit": "node-iters/s", "ms_per_step": 1e3 * t_host,
                            "note": "phi in pageable host memory: copy in, reinitialize, copy out"},

@@ -158,6 +158,27 @@ namespace {
 
 // reinitialize_within_range_of_phi_0 driver, templated on dimension and the
 // stage arithmetic type R (double: FP64, float: mixed).
+// Every scratch buffer a reinitialization call of this kind uses, allocated
+// up front (a recorded graph must not allocate).
+template <int D, class R>
+void reinit_scratch(casl_stencil *st, bool so, bool tiled) {
+  const size_t N = size_t(st->N);
+  constexpr int F = 2 * D;
+  st->phi0.ensure(N);
+  st->flags.ensure(N);
+  if (so && !tiled) st->dd0.ensure(size_t(D) * N);
+  if constexpr (sizeof(R) == 8) {
+    st->sfix64.ensure(size_t(F) * N);
+    st->tmp64.ensure(N);
+    if (so) st->ddn64.ensure(size_t(D) * N);
+  } else {
+    st->sfixf.ensure(size_t(F) * N);
+    st->phif.ensure(N);
+    st->tmpf.ensure(N);
+    if (so) st->ddnf.ensure(size_t(D) * N);
+  }
+}
+
 template <int D, class R>
 void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_space, int order_time, int n_iter,
                 double limit_high, double limit_low, hipStream_t s) {
@@ -170,11 +191,9 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   const unsigned grid = grid_for(N);
 
   const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
-  st->phi0.ensure(N);
-  st->flags.ensure(N);
+  reinit_scratch<D, R>(st, so, tiled);
   HIP_CHECK(hipMemcpyAsync(st->phi0.p, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   if (so && !tiled) {
-    st->dd0.ensure(size_t(D) * N);
     prof_begin("k_dd", s);
     hipLaunchKernelGGL((k_dd<D, double>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p);
     prof_end("k_dd", s);
@@ -182,18 +201,11 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   }
   R *sfix, *cur, *nxt, *ddn;
   if constexpr (sizeof(R) == 8) {
-    st->sfix64.ensure(size_t(F) * N);
-    st->tmp64.ensure(N);
-    if (so) st->ddn64.ensure(size_t(D) * N);
     sfix = st->sfix64.p;
     cur = phi;
     nxt = st->tmp64.p;
     ddn = st->ddn64.p;
   } else {
-    st->sfixf.ensure(size_t(F) * N);
-    st->phif.ensure(N);
-    st->tmpf.ensure(N);
-    if (so) st->ddnf.ensure(size_t(D) * N);
     sfix = st->sfixf.p;
     cur = st->phif.p;
     nxt = st->tmpf.p;
@@ -470,11 +482,14 @@ Profiler &profiler() {
 }
 }  // namespace
 
+// Set while a call is being recorded into a graph: no timing events then.
+thread_local bool g_recording = false;
+
 bool prof_on() { return profiler().on.load(std::memory_order_relaxed); }
 
 void prof_begin(const char *kernel, hipStream_t s) {
   Profiler &P = profiler();
-  if (!P.on) return;
+  if (!P.on || g_recording) return;
   std::lock_guard<std::mutex> g(P.mu);
   Profiler::Pending p;
   HIP_CHECK(hipEventCreate(&p.a));
@@ -485,7 +500,7 @@ void prof_begin(const char *kernel, hipStream_t s) {
 
 void prof_end(const char *kernel, hipStream_t s) {
   Profiler &P = profiler();
-  if (!P.on) return;
+  if (!P.on || g_recording) return;
   std::lock_guard<std::mutex> g(P.mu);
   auto it = P.open.find(kernel);
   if (it == P.open.end()) return;
@@ -582,29 +597,50 @@ int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd, vo
   });
 }
 
+static void check_reinit_args(casl_stencil_t *st, double *phi, int order_space, int order_time, int n_iter,
+                              double limit_high, double limit_low, int precision) {
+  if (!st || !phi) throw std::invalid_argument("null argument");
+  if (order_space == 0 || order_space > 2 || order_time == 0 || order_time > 2)
+    throw std::invalid_argument(
+        "reinitialize_within_range_of_phi_0: valid orders of accuracy for space and pseudotime are 1 and 2 only!");
+  if (n_iter < 0) throw std::invalid_argument("negative number of iterations");
+  if (!(limit_high > limit_low)) throw std::invalid_argument("limit_high must exceed limit_low");
+  if (precision != CASL_PRECISION_FP64 && precision != CASL_PRECISION_MIXED)
+    throw std::invalid_argument("unknown precision mode");
+}
+
+// prepare_only: allocate the call's scratch, launch nothing
+static void dispatch_reinit(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space, int order_time,
+                            int n_iter, double limit_high, double limit_low, int precision, hipStream_t s,
+                            bool prepare_only = false) {
+  const bool so = order_space > 1, tiled = st->tiled && st->path == CASL_PATH_TILED;
+  const bool fp64 = precision == CASL_PRECISION_FP64;
+  if (prepare_only) {
+    if (st->dim == 2)
+      fp64 ? reinit_scratch<2, double>(st, so, tiled) : reinit_scratch<2, float>(st, so, tiled);
+    else
+      fp64 ? reinit_scratch<3, double>(st, so, tiled) : reinit_scratch<3, float>(st, so, tiled);
+    return;
+  }
+  if (st->dim == 2) {
+    if (fp64)
+      run_reinit<2, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+    else
+      run_reinit<2, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+  } else {
+    if (fp64)
+      run_reinit<3, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+    else
+      run_reinit<3, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+  }
+}
+
 static int reinit_common(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space, int order_time,
                          int n_iter, double limit_high, double limit_low, int precision, void *stream) {
   return guarded([&]() {
-    if (!st || !phi) throw std::invalid_argument("null argument");
-    if (order_space == 0 || order_space > 2 || order_time == 0 || order_time > 2)
-      throw std::invalid_argument(
-          "reinitialize_within_range_of_phi_0: valid orders of accuracy for space and pseudotime are 1 and 2 only!");
-    if (n_iter < 0) throw std::invalid_argument("negative number of iterations");
-    if (!(limit_high > limit_low)) throw std::invalid_argument("limit_high must exceed limit_low");
-    if (precision != CASL_PRECISION_FP64 && precision != CASL_PRECISION_MIXED)
-      throw std::invalid_argument("unknown precision mode");
-    hipStream_t s = static_cast<hipStream_t>(stream);
-    if (st->dim == 2) {
-      if (precision == CASL_PRECISION_FP64)
-        run_reinit<2, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
-      else
-        run_reinit<2, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
-    } else {
-      if (precision == CASL_PRECISION_FP64)
-        run_reinit<3, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
-      else
-        run_reinit<3, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
-    }
+    check_reinit_args(st, phi, order_space, order_time, n_iter, limit_high, limit_low, precision);
+    dispatch_reinit(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, precision,
+                    static_cast<hipStream_t>(stream));
   });
 }
 
@@ -621,6 +657,55 @@ int casl_reinit_masked(casl_stencil_t *st, double *phi, const uint8_t *mask, int
     return CASL_ERR_INVALID_ARGUMENT;
   }
   return reinit_common(st, phi, mask, 2, 2, n_iter, limit_high, limit_low, precision, stream);
+}
+
+int casl_reinit_graph(casl_stencil_t *st, double *phi, int order_space, int order_pseudotime, int n_iter,
+                      double limit_high, double limit_low, int precision, void *stream) {
+  return guarded([&]() {
+    check_reinit_args(st, phi, order_space, order_pseudotime, n_iter, limit_high, limit_low, precision);
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // what the recording depends on: the call's arguments, the stage path and
+    // every scratch address the recorded kernels read or write
+    dispatch_reinit(st, phi, nullptr, order_space, order_pseudotime, n_iter, limit_high, limit_low, precision, s,
+                    true);
+    uint64_t lh, ll;
+    std::memcpy(&lh, &limit_high, sizeof(lh));
+    std::memcpy(&ll, &limit_low, sizeof(ll));
+    const std::vector<uintptr_t> key = {
+        uintptr_t(phi), uintptr_t(order_space), uintptr_t(order_pseudotime), uintptr_t(n_iter), uintptr_t(lh),
+        uintptr_t(ll), uintptr_t(precision), uintptr_t(st->path), uintptr_t(st->phi0.p), uintptr_t(st->flags.p),
+        uintptr_t(st->dd0.p), uintptr_t(st->sfix64.p), uintptr_t(st->tmp64.p), uintptr_t(st->ddn64.p),
+        uintptr_t(st->sfixf.p), uintptr_t(st->phif.p), uintptr_t(st->tmpf.p), uintptr_t(st->ddnf.p)};
+    if (!st->graph_exec || st->graph_key != key) {
+      st->drop_graph();
+      hipStream_t cap;
+      HIP_CHECK(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+      g_recording = true;
+      try {
+        HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+        try {
+          dispatch_reinit(st, phi, nullptr, order_space, order_pseudotime, n_iter, limit_high, limit_low, precision,
+                          cap);
+        } catch (...) {
+          hipGraph_t broken = nullptr;
+          (void)hipStreamEndCapture(cap, &broken);
+          if (broken) (void)hipGraphDestroy(broken);
+          throw;
+        }
+        HIP_CHECK(hipStreamEndCapture(cap, &st->graph));
+        HIP_CHECK(hipGraphInstantiate(&st->graph_exec, st->graph, nullptr, nullptr, 0));
+      } catch (...) {
+        g_recording = false;
+        (void)hipStreamDestroy(cap);
+        st->drop_graph();
+        throw;
+      }
+      g_recording = false;
+      HIP_CHECK(hipStreamDestroy(cap));
+      st->graph_key = key;
+    }
+    HIP_CHECK(hipGraphLaunch(st->graph_exec, s));
+  });
 }
 
 int casl_band_linf(casl_stencil_t *st, const double *a, const double *b, const double *ref, double band,

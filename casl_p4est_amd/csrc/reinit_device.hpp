@@ -11,6 +11,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/casl_reinit.h"
 #include "stencil_math.hpp"
@@ -403,6 +404,22 @@ struct casl_stencil {
   casl::DevBuf<uint16_t> t_rec_slot;
   casl::DevBuf<uint4> t_rec;
   casl::DevBuf<uint16_t> t_ext;
+  // casl_reinit_graph: the recorded call and what it was recorded for
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<uintptr_t> graph_key;
+
+  casl_stencil() = default;
+  casl_stencil(const casl_stencil &) = delete;
+  casl_stencil &operator=(const casl_stencil &) = delete;
+  ~casl_stencil() { drop_graph(); }
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_key.clear();
+  }
 
   template <int D>
   casl::dev::Tables<D> tables() const {

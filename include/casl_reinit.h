@@ -148,6 +148,16 @@ int casl_reinit(casl_stencil_t *st, double *phi, int order_space,
                 int order_pseudotime, int n_iter, double limit_high,
                 double limit_low, int precision, void *stream);
 
+/* casl_reinit recorded into a hipGraph on its first call and replayed on
+ * later calls with the same arguments (phi address included) and the same
+ * stage path: one graph launch instead of 2 + 2 * n_iter kernel launches,
+ * which matters for small grids.  Results equal casl_reinit's.  A call with
+ * other arguments records again.  No counterpart in the reference (the
+ * reinitialize_* methods are host loops over PETSc calls). */
+int casl_reinit_graph(casl_stencil_t *st, double *phi, int order_space,
+                      int order_pseudotime, int n_iter, double limit_high,
+                      double limit_low, int precision, void *stream);
+
 /* reinitialize_2nd_order_with_mask: nodes with mask[n] == 0 keep their
  * current value at every stage (orders 2/2).  mask is a device uint8 array. */
 int casl_reinit_masked(casl_stencil_t *st, double *phi, const uint8_t *mask,

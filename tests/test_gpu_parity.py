@@ -289,3 +289,41 @@ def test_advect_in_normal_direction_matches_oracle(cuda, name):
     # the fused tiled stages and the grid-wide kernels: the same bits
     assert np.array_equal(out["tiled"][0], out["two_pass"][0])
     assert np.array_equal(out["tiled"][1], out["two_pass"][1])
+
+
+@pytest.mark.parametrize("name", sorted(PROBLEMS))
+def test_reinit_graph_replays_equal_eager(cuda, name):
+    """casl_reinit_graph: every replay equals the eager call bit for bit, with
+    eager work of the same stencil in between; other arguments or another
+    stage path record again."""
+    import torch
+    dim, shape, lmin, lmax = PROBLEMS[name]
+    say(f"[gpu] graph replay {name}")
+    f, ngbd, phi0 = problem(dim, shape, lmin, lmax)
+    ds = ngbd.device()
+    p0 = torch.from_numpy(phi0).to(cuda)
+    for prec in (0, 1):
+        for n_iter in (7, 3):
+            eager = p0.clone()
+            ds.reinit(eager, 2, 2, n_iter, C.DBL_MAX, -C.DBL_MAX, prec)
+            g = torch.empty_like(p0)
+            other = p0.clone()
+            for rep in range(3):
+                g.copy_(p0)
+                ds.reinit(g, 2, 2, n_iter, C.DBL_MAX, -C.DBL_MAX, prec, graph=True)
+                torch.cuda.synchronize()
+                assert torch.equal(g, eager), (prec, n_iter, rep)
+                # eager work on the same stencil (same scratch) between replays
+                other.copy_(p0)
+                ds.reinit(other, 2, 1, 2, C.DBL_MAX, -C.DBL_MAX, 1 - prec)
+                torch.cuda.synchronize()
+    ds.set_path("two_pass")
+    eager = p0.clone()
+    ds.reinit(eager, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0)
+    g = p0.clone()
+    ds.reinit(g, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0, graph=True)
+    torch.cuda.synchronize()
+    assert torch.equal(g, eager)
+    ds.set_path("tiled")
+    with pytest.raises(C.CaslError):
+        ds.reinit(g, 3, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0, graph=True)

@@ -92,7 +92,7 @@ def test_sample_matches_levelset_eval():
     _, xyz = f.nodes()
     star = ls.Star()
     assert np.array_equal(f.sample(star), star(xyz))
-    # python restatement of the star formula (geom::Star-like), config 1
+    # python restatement of the star formula (geom::Star-like): r = 0.3 + 0.05 cos 5t, (1 + 0.8 |x - c|^2)
     dx, dy = xyz[:, 0] - 0.5, xyz[:, 1] - 0.5
     r = 0.3 + 0.05 * np.cos(5 * np.arctan2(dy, dx))
     ref = (np.sqrt(dx * dx + dy * dy) - r) * (1 + 0.8 * (dx * dx + dy * dy))

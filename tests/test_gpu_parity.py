@@ -137,10 +137,10 @@ def test_golden_fixtures(cuda, name, path):
         assert rel_err(p.cpu().numpy(), z[f"order{os_}{ot}_5"]) <= REL
 
 
-def test_config1_star_fp64_matches_oracle(cuda):
+def test_star2d_lmax9_fp64_matches_oracle(cuda):
     """2D star lmin 4 / lmax 9, 20 RK2 iterations, FP64 (the CPU-runnable config)."""
     import torch
-    say("[gpu] config 1 star lmin4/lmax9 x20")
+    say("[gpu] 2D star lmin 4 / lmax 9, 20 iterations")
     f, ngbd, phi0 = problem(2, ls.Star(), 4, 9)
     ref = O.reinit(ngbd.tables, phi0, 2, 2, 20)
     phi = torch.from_numpy(phi0).to(cuda)

@@ -138,7 +138,7 @@ def test_golden_fixtures(cuda, name, path):
 
 
 def test_star2d_lmax9_fp64_matches_oracle(cuda):
-    """2D star lmin 4 / lmax 9, 20 RK2 iterations, FP64 (the CPU-runnable config)."""
+    """2D star lmin 4 / lmax 9, 20 RK2 iterations, FP64 (small enough for a one-core oracle run)."""
     import torch
     say("[gpu] 2D star lmin 4 / lmax 9, 20 iterations")
     f, ngbd, phi0 = problem(2, ls.Star(), 4, 9)

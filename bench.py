@@ -565,8 +565,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the CPU baseline (default: 1 for star2d, all cores otherwise)")
-    ap.add_argument("--path", default="tiled", choices=["tiled", "two_pass"],
-                    help="stage kernels: fused tiled kernel (default) or grid-wide two-pass")
+    ap.add_argument("--path", default="auto", choices=["auto", "tiled", "two_pass"],
+                    help="stage kernels: fused tiled kernel, grid-wide two-pass, or the library's choice (default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this build has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -593,7 +593,10 @@ def main():
     N, dim, iters = W["ngbd"].num_nodes, W["dim"], W["iters"]
     h = float(np.min(W["forest"].dxyz_min()))
     lsr = C.LevelSet(W["ngbd"])
-    W["ngbd"].device().set_path(args.path)  # uploads tables before timing
+    dev_st = W["ngbd"].device()  # uploads tables before timing
+    if args.path != "auto":
+        dev_st.set_path(args.path)
+    args.path = dev_st.path
     log(f"[bench] stencil tiles: {W['ngbd'].device().tile_info()}")
     phi0 = torch.from_numpy(W["phi0"]).to(dev)
     phi = torch.empty_like(phi0)

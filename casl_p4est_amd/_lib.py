@@ -77,6 +77,7 @@ SIGNATURES = {
     "casl_last_error": (ctypes.c_char_p, []),
     "casl_stencil_create": (ctypes.c_int, [ctypes.POINTER(FlatStencil), ctypes.POINTER(ctypes.c_void_p)]),
     "casl_stencil_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "casl_stencil_get_path": (ctypes.c_int, [ctypes.c_void_p, c_int32_p]),
     "casl_stencil_info": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p]),
     "casl_stencil_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "casl_stencil_tile_info": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p, c_int64_p, c_int64_p,

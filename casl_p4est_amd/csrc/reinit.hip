@@ -414,6 +414,13 @@ void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
   }
   validate_tile_host(H, tab->dim, tab->n_nodes);
   upload_tile_host(H, tab->n_nodes, st);
+  // One block per tile: a grid of fewer tiles than compute units leaves most
+  // of the chip idle, and there the grid-wide kernels (more, smaller blocks)
+  // measured faster (the 15-tile 2D star: 9.5e8 against 7.7e8 FP64
+  // node-iterations/s).  casl_stencil_set_path overrides the choice.
+  hipDeviceProp_t prop;
+  HIP_CHECK(hipGetDeviceProperties(&prop, st->device));
+  st->path = st->n_tiles >= int64_t(prop.multiProcessorCount) ? CASL_PATH_TILED : CASL_PATH_TWO_PASS;
 }
 
 void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles) {
@@ -561,6 +568,13 @@ int casl_stencil_set_path(casl_stencil_t *st, int path) {
     if (!st) throw std::invalid_argument("null stencil");
     if (path != CASL_PATH_TILED && path != CASL_PATH_TWO_PASS) throw std::invalid_argument("unknown path");
     st->path = path;
+  });
+}
+
+int casl_stencil_get_path(const casl_stencil_t *st, int32_t *path) {
+  return guarded([&]() {
+    if (!st || !path) throw std::invalid_argument("null argument");
+    *path = st->path;
   });
 }
 

@@ -56,10 +56,17 @@ class DeviceStencil:
         return self._h
 
     def set_path(self, path):
-        """"tiled" (fused stage kernel over LDS-staged Morton tiles, default when
-        the stencil could be tiled) or "two_pass" (grid-wide pass A + pass B)."""
+        """"tiled" (fused stage kernel over LDS-staged Morton tiles; the default
+        when the stencil makes at least one tile per compute unit) or
+        "two_pass" (grid-wide pass A + pass B; the default for smaller grids)."""
         code = {"tiled": L.CASL_PATH_TILED, "two_pass": L.CASL_PATH_TWO_PASS}[path]
         L.call("casl_stencil_set_path", self._h, code)
+
+    @property
+    def path(self):
+        v = ctypes.c_int32()
+        L.call("casl_stencil_get_path", self._h, ctypes.byref(v))
+        return "tiled" if v.value == L.CASL_PATH_TILED else "two_pass"
 
     def tile_info(self):
         vals = [ctypes.c_int32()] + [ctypes.c_int64() for _ in range(4)]

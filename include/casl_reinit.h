@@ -91,12 +91,15 @@ int casl_stencil_create(const casl_flat_stencil *tables, casl_stencil_t **out);
 int casl_stencil_destroy(casl_stencil_t *st);
 int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes);
 
-/* Stage kernel selection.  CASL_PATH_TILED (default when the stencil could
- * be tiled): one fused kernel per pseudo-time stage over Morton tiles with
- * LDS-staged halos; CASL_PATH_TWO_PASS: grid-wide pass A + pass B kernels. */
+/* Stage kernel selection.  CASL_PATH_TILED: one fused kernel per
+ * pseudo-time stage over Morton tiles with LDS-staged halos (the default when
+ * the stencil could be tiled into at least one tile per compute unit);
+ * CASL_PATH_TWO_PASS: grid-wide pass A + pass B kernels (the default for
+ * smaller grids). */
 #define CASL_PATH_TILED    0
 #define CASL_PATH_TWO_PASS 1
 int casl_stencil_set_path(casl_stencil_t *st, int path);
+int casl_stencil_get_path(const casl_stencil_t *st, int32_t *path);
 /* tiled: 1 if the tiled encoding exists; counts of tiles, halo entries
  * (H1 + H2 over all tiles), records (T + H1 over all tiles) and records of
  * nodes with a hanging arm. */

@@ -212,6 +212,13 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
   const R m21 = div_dsum<R>(t, div_d<R>(t, R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)), 2, n), 1, n) +
                 div_dsum<R>(t, div_d<R>(t, R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)), 3, n), 1, n);
   if constexpr (D == 2) {
+    if constexpr (sizeof(R) == 4) {  // FP32: the general inverse, as in 3D below
+      const R idet = qdiv(R(1.0), R(1.0) - m12 * m21);
+      const R nDxx = Dxx, nDyy = Dyy;
+      Dxx = nDxx * idet - nDyy * (m12 * idet);
+      Dyy = nDyy * idet - nDxx * (m21 * idet);
+      return;
+    }
     if (check_if_zero(m12)) {
       if (!check_if_zero(m21)) Dyy -= m21 * Dxx;
     } else if (check_if_zero(m21)) {
@@ -231,6 +238,22 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
                   div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)), 5, n), 2, n);
     const R m32 = div_dsum<R>(t, div_d<R>(t, R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)), 4, n), 2, n) +
                   div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)), 5, n), 2, n);
+    if constexpr (sizeof(R) == 4) {
+      // FP32 (the mixed path): the general inverse for every case.  The m_ij
+      // are ratios of dyadic lengths, so each is exactly 0 or far above EPS,
+      // and the reference's zero tests only pick which terms vanish; without
+      // the branch tree the lanes of a wave no longer split over its cases.
+      const R idet =
+          qdiv(R(1.0), R(1.0) + m12 * m23 * m31 + m13 * m21 * m32 - m12 * m21 - m13 * m31 - m23 * m32);
+      const R nDxx = Dxx, nDyy = Dyy, nDzz = Dzz;
+      Dxx = (nDxx * (idet * (R(1.0) - m23 * m32)) + nDyy * (idet * (m32 * m13 - m12)) +
+             nDzz * (idet * (m12 * m23 - m13)));
+      Dyy = (nDxx * (idet * (m31 * m23 - m21)) + nDyy * (idet * (R(1.0) - m31 * m13)) +
+             nDzz * (idet * (m21 * m13 - m23)));
+      Dzz = (nDxx * (idet * (m21 * m32 - m31)) + nDyy * (idet * (m31 * m12 - m32)) +
+             nDzz * (idet * (R(1.0) - m21 * m12)));
+      return;
+    }
     if (check_if_zero(m12) && check_if_zero(m13)) {
       if (check_if_zero(m23)) {
         const bool z21 = !check_if_zero(m21), z31 = !check_if_zero(m31), z32 = !check_if_zero(m32);

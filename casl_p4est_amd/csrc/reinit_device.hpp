@@ -279,9 +279,9 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
 #pragma unroll
     for (int a = 0; a < D; ++a) {
       const R ddm = minmod(pn_dd_f[2 * a], pn_dd_000[a]);
-      Dm[a] += R(0.5) * s[2 * a] * ddm;
+      Dm[a] = qfma(R(0.5) * s[2 * a], ddm, Dm[a]);
       const R ddp = minmod(pn_dd_f[2 * a + 1], pn_dd_000[a]);
-      Dp[a] -= R(0.5) * s[2 * a + 1] * ddp;
+      Dp[a] = qfma(R(-0.5) * s[2 * a + 1], ddp, Dp[a]);
     }
   }
   // local time step and Godunov upwinding (:250-271)
@@ -304,7 +304,7 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
   }
   R sum = fmax(Dp[0] * Dp[0], Dm[0] * Dm[0]) + fmax(Dp[1] * Dp[1], Dm[1] * Dm[1]);
   if constexpr (D == 3) sum = sum + fmax(Dp[2] * Dp[2], Dm[2] * Dm[2]);
-  R r = pn_000 - dt * sgn * (qsqrt(sum) - R(1.));
+  R r = qfma(-(dt * sgn), qsqrt(sum) - R(1.), pn_000);
   // sign guard phi_0*phi_np1 < 0 -> flip (:273-274); phi_0 != 0 for updated nodes
   if ((fl & 4) ? (r < R(0.0)) : (r > R(0.0))) r *= R(-1.0);
   return r;

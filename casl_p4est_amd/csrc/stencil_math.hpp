@@ -102,6 +102,17 @@ __device__ __forceinline__ float qsqrt<float>(float x) {
   return __builtin_amdgcn_sqrtf(x);
 }
 
+// a * b + c: one fused operation in FP32 (the mixed path); FP64 keeps the
+// separately rounded product and sum of the reference expression.
+template <class R>
+__device__ __forceinline__ R qfma(R a, R b, R c) {
+  return a * b + c;
+}
+template <>
+__device__ __forceinline__ float qfma<float>(float a, float b, float c) {
+  return __builtin_fmaf(a, b, c);
+}
+
 // Exact-inverse division.  A stencil accessor with TB::FASTDIV and t.fast
 // has power-of-two face distances d_f = 1 / iq[f] (cubic dyadic cells of a
 // power-of-two tree, no wall arm).  Dividing by a power of two and
@@ -154,7 +165,7 @@ __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G 
     const R w_mm = d_p0;
     const R w_pm = d_m0;
     const R normalization = (d_m0 + d_p0);
-    const R num = R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 0, n)]) * w_mm;
+    const R num = qfma(R(g[t.idx(f, 0, n)]), w_mm, R(g[t.idx(f, 1, n)]) * w_pm);
     // d_m0 + d_p0 is the arm cell's transverse size, which equals d_f here
     if constexpr (irregular_fastdiv<R, TB>()) {
       if (t.fast) return num * t.iq[f];
@@ -167,8 +178,8 @@ __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G 
     const R w_mp = d_p0 * d_0m;
     const R w_pp = d_m0 * d_0m;
     const R normalization = (d_m0 + d_p0) * (d_0m + d_0p);
-    const R num = R(g[t.idx(f, 3, n)]) * w_pp + R(g[t.idx(f, 1, n)]) * w_pm + R(g[t.idx(f, 2, n)]) * w_mp +
-                  R(g[t.idx(f, 0, n)]) * w_mm;
+    const R num = qfma(R(g[t.idx(f, 0, n)]), w_mm,
+                       qfma(R(g[t.idx(f, 2, n)]), w_mp, qfma(R(g[t.idx(f, 1, n)]), w_pm, R(g[t.idx(f, 3, n)]) * w_pp)));
     // both transverse sums are the arm cell's size d_f (cubic cells)
     if constexpr (irregular_fastdiv<R, TB>()) {
       if (t.fast) return num * (t.iq[f] * t.iq[f]);
@@ -193,7 +204,7 @@ __device__ __forceinline__ R second_difference(const TB &t, int a, R fp, R f0, R
   const int fpp = 2 * a + 1, fmm = 2 * a;
   if constexpr (TB::FASTDIV) {
     if (t.fast) {
-      const R x = ((fp - f0) * t.iq[fpp] + (fm - f0) * t.iq[fmm]) * R(2.);
+      const R x = qfma(fp - f0, t.iq[fpp], (fm - f0) * t.iq[fmm]) * R(2.);
       return t.dist[fpp] == t.dist[fmm] ? x * (R(0.5) * t.iq[fpp]) : qdiv(x, t.dist[fpp] + t.dist[fmm]);
     }
   }
@@ -215,8 +226,8 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
     if constexpr (sizeof(R) == 4) {  // FP32: the general inverse, as in 3D below
       const R idet = qdiv(R(1.0), R(1.0) - m12 * m21);
       const R nDxx = Dxx, nDyy = Dyy;
-      Dxx = nDxx * idet - nDyy * (m12 * idet);
-      Dyy = nDyy * idet - nDxx * (m21 * idet);
+      Dxx = qfma(-nDyy, m12 * idet, nDxx * idet);
+      Dyy = qfma(-nDxx, m21 * idet, nDyy * idet);
       return;
     }
     if (check_if_zero(m12)) {
@@ -246,12 +257,9 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
       const R idet =
           qdiv(R(1.0), R(1.0) + m12 * m23 * m31 + m13 * m21 * m32 - m12 * m21 - m13 * m31 - m23 * m32);
       const R nDxx = Dxx, nDyy = Dyy, nDzz = Dzz;
-      Dxx = (nDxx * (idet * (R(1.0) - m23 * m32)) + nDyy * (idet * (m32 * m13 - m12)) +
-             nDzz * (idet * (m12 * m23 - m13)));
-      Dyy = (nDxx * (idet * (m31 * m23 - m21)) + nDyy * (idet * (R(1.0) - m31 * m13)) +
-             nDzz * (idet * (m21 * m13 - m23)));
-      Dzz = (nDxx * (idet * (m21 * m32 - m31)) + nDyy * (idet * (m31 * m12 - m32)) +
-             nDzz * (idet * (R(1.0) - m21 * m12)));
+      Dxx = qfma(nDzz, idet * (m12 * m23 - m13), qfma(nDyy, idet * (m32 * m13 - m12), nDxx * (idet * (R(1.0) - m23 * m32))));
+      Dyy = qfma(nDzz, idet * (m21 * m13 - m23), qfma(nDyy, idet * (R(1.0) - m31 * m13), nDxx * (idet * (m31 * m23 - m21))));
+      Dzz = qfma(nDzz, idet * (R(1.0) - m21 * m12), qfma(nDyy, idet * (m31 * m12 - m32), nDxx * (idet * (m21 * m32 - m31))));
       return;
     }
     if (check_if_zero(m12) && check_if_zero(m13)) {
@@ -348,12 +356,12 @@ __device__ __forceinline__ void quadratic_corrections(const TB &t, int64_t n, R 
     fnb[2] -= (h * R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)) * dd[0]);
     fnb[3] -= (h * R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)) * dd[0]);
   } else {
-    fnb[0] -= (h * R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)) * dd[1] + h * R(t.dt(0, 2, n)) * R(t.dt(0, 3, n)) * dd[2]);
-    fnb[1] -= (h * R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)) * dd[1] + h * R(t.dt(1, 2, n)) * R(t.dt(1, 3, n)) * dd[2]);
-    fnb[2] -= (h * R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)) * dd[0] + h * R(t.dt(2, 2, n)) * R(t.dt(2, 3, n)) * dd[2]);
-    fnb[3] -= (h * R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)) * dd[0] + h * R(t.dt(3, 2, n)) * R(t.dt(3, 3, n)) * dd[2]);
-    fnb[4] -= (h * R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)) * dd[0] + h * R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)) * dd[1]);
-    fnb[5] -= (h * R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)) * dd[0] + h * R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)) * dd[1]);
+    // fnb[f] -= (h d_m0 d_p0 f_tt1 + h d_0m d_0p f_tt2), the first product fused in FP32
+    constexpr int t1[6] = {1, 1, 0, 0, 0, 0}, t2[6] = {2, 2, 2, 2, 1, 1};
+#pragma unroll
+    for (int f = 0; f < 6; ++f)
+      fnb[f] -= qfma(h * R(t.dt(f, 0, n)) * R(t.dt(f, 1, n)), dd[t1[f]],
+                     h * R(t.dt(f, 2, n)) * R(t.dt(f, 3, n)) * dd[t2[f]]);
   }
 }
 

@@ -574,7 +574,7 @@ int casl_stencil_set_path(casl_stencil_t *st, int path) {
 int casl_stencil_get_path(const casl_stencil_t *st, int32_t *path) {
   return guarded([&]() {
     if (!st || !path) throw std::invalid_argument("null argument");
-    *path = st->path;
+    *path = st->tiled ? st->path : CASL_PATH_TWO_PASS;
   });
 }
 

@@ -95,7 +95,10 @@ int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes);
  * pseudo-time stage over Morton tiles with LDS-staged halos (the default when
  * the stencil could be tiled into at least one tile per compute unit);
  * CASL_PATH_TWO_PASS: grid-wide pass A + pass B kernels (the default for
- * smaller grids). */
+ * smaller grids).  A stencil without a tiled encoding (cells that are not
+ * cubes, or tile limits exceeded; casl_stencil_tile_info gives the reason)
+ * runs CASL_PATH_TWO_PASS whatever was set; get_path returns the path that
+ * runs. */
 #define CASL_PATH_TILED    0
 #define CASL_PATH_TWO_PASS 1
 int casl_stencil_set_path(casl_stencil_t *st, int path);

@@ -82,6 +82,38 @@ def test_reinit_fp64_matches_oracle(cuda, name, orders, path):
     say(f"    max |gpu - oracle| = {np.max(np.abs(out - ref)):.3e}  identical={np.array_equal(out, ref)}")
 
 
+@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5])], ids=["rect2d", "brick3d"])
+@pytest.mark.parametrize("path", ["tiled", "two_pass"])
+def test_noncubic_box_matches_oracle(cuda, dim, box_max, path):
+    """A single tree over a box whose edges differ (by powers of two): cells
+    are not cubes, so a hanging arm's transverse sums differ from its length.
+    The tile encoding assumes cubic cells and is not built for such a grid;
+    asking for the tiled path then runs the grid-wide tables, with the
+    oracle's results."""
+    import torch
+    say(f"[gpu] non-cubic box {box_max} path={path}")
+    shape = ls.Sphere([0.9] + [0.45] * (dim - 1), 0.3) if dim == 2 else ls.Sphere([0.9, 0.5, 0.25], 0.2)
+    f = C.Forest(dim, [0.0] * dim, box_max).refine_levelset(shape, 2, 7 if dim == 2 else 6, lip=2.4).build_nodes()
+    ngbd = C.NodeNeighbors(f).init_neighbors()
+    phi0 = f.sample(shape)
+    info = ngbd.device().tile_info()
+    assert not info["tiled"], info
+    ngbd.device().set_path(path)
+    assert ngbd.device().path == "two_pass"
+    it = 10
+    ref = O.reinit(ngbd.tables, phi0, 2, 2, it)
+    phi = torch.from_numpy(phi0).to(cuda)
+    C.LevelSet(ngbd).reinitialize_within_range_of_phi_0(phi, 2, 2, C.DBL_MAX, -C.DBL_MAX, it)
+    out = phi.cpu().numpy()
+    say(f"    max |gpu - oracle| = {np.max(np.abs(out - ref)):.3e}")
+    assert rel_err(out, ref) <= REL
+    h = float(np.min(f.dxyz_min()))
+    mixed = torch.from_numpy(phi0).to(cuda)
+    C.LevelSet(ngbd, precision="mixed").reinitialize_within_range_of_phi_0(mixed, 2, 2, C.DBL_MAX, -C.DBL_MAX, it)
+    band = np.abs(out) < 3 * h
+    assert np.max(np.abs(mixed.cpu().numpy() - out)[band]) <= 1e-3 * h
+
+
 @pytest.mark.parametrize("path", ["tiled", "two_pass"])
 def test_reinit_limits_and_mask(cuda, path):
     import torch

@@ -62,6 +62,9 @@ __global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu
   const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
   const int nH1 = A.h1count[tile];
   const int rb = A.rec_start[tile];
+  const int ext0 = A.ext_start[tile];
+  const uint32_t nreg = A.nreg[tile];
+  constexpr int XW = HangExt<D>::WORDS;
   TileRec trec[KT];
   int tslot[KT];
   double tvn[KT], tprev[KT];
@@ -82,24 +85,27 @@ __global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu
       if constexpr (RK_AVG) tprev[k] = A.out[s + tslot[k]];
     }
   __syncthreads();
-  auto pass_a = [&](const TileRec &rec, int j) {
+  // rec: the record at tile-local position i, j: its LDS slot
+  auto pass_a = [&](const TileRec &rec, int j, int i) {
     double f000, fnb[2 * D], dd[D];
     if (!(rec.geo & GEO_IRREGULAR)) {
       RegStencil<D, double> st;
       st.load(rec, A.geom);
       laplace<D, double>(st, j, phis, f000, fnb, dd);
     } else {
-      IrrStencil<D, double> st;
-      st.load(rec, A.ext, A.geom);
+      HangStencil<D, double> st;
+      st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
       laplace<D, double>(st, j, phis, f000, fnb, dd);
     }
 #pragma unroll
     for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
   };
 #pragma unroll
-  for (int k = 0; k < KT; ++k)
-    if (int(threadIdx.x) + k * TILE_BLOCK < nT) pass_a(trec[k], tslot[k]);
-  for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
+  for (int k = 0; k < KT; ++k) {
+    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    if (i < nT) pass_a(trec[k], tslot[k], i);
+  }
+  for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i], i);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
@@ -113,8 +119,8 @@ __global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu
       st.load(rec, A.geom);
       r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
     } else {
-      IrrStencil<D, double> st;
-      st.load(rec, A.ext, A.geom);
+      HangStencil<D, double> st;
+      st.load(rec, hang_record(A.ext, ext0, nreg, nT, int(threadIdx.x) + k * TILE_BLOCK, XW), A.geom);
       r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
     }
     if constexpr (RK_AVG)

@@ -347,7 +347,9 @@ void launch_batch(casl_batch *B, double *phi, int n_iter, hipStream_t s, bool al
 // concatenated: node, halo, record and extension indices offset per tree.
 void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std::vector<int64_t> &off) {
   const int32_t nt = B->n_trees;
-  const int D = trees[0].dim, F = 2 * D, W = (D == 2 ? 3 : 6);
+  const int D = trees[0].dim;
+  const int WORDS = D == 2 ? HangExt<2>::WORDS : HangExt<3>::WORDS, SW = D == 2 ? HangExt<2>::SW : HangExt<3>::SW;
+  const int NH = D == 2 ? HangExt<2>::NH : HangExt<3>::NH;
   std::vector<TileHost> hosts(static_cast<size_t>(nt));
   std::vector<std::string> errs(static_cast<size_t>(nt));
   parallel_for(nt, [&](int64_t b, int64_t e) {
@@ -381,6 +383,7 @@ void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std:
   all.tile_start.push_back(0);
   all.halo_start.push_back(0);
   all.rec_start.push_back(0);
+  all.ext_start.push_back(0);
   std::vector<int32_t> tree_tile(static_cast<size_t>(nt) + 1, 0);
   const int C = 1 << (D - 1);
   for (int32_t k = 0; k < nt; ++k) {
@@ -388,7 +391,7 @@ void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std:
     const int32_t node_off = int32_t(off[size_t(k)]);
     const int32_t halo_off = int32_t(all.halo_idx.size());
     const int32_t rec_off = int32_t(all.rec.size());
-    const uint32_t ext_off = uint32_t(all.ext.size() / size_t(F * W));
+    const int32_t ext_off = int32_t(all.ext.size() / size_t(WORDS));
     const int64_t ntile = int64_t(H.tile_start.size()) - 1;
     tree_tile[size_t(k)] = int32_t(all.tile_start.size()) - 1;
     for (int64_t t = 0; t < ntile; ++t) {
@@ -396,21 +399,17 @@ void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std:
       all.halo_start.push_back(H.halo_start[size_t(t) + 1] + halo_off);
       all.rec_start.push_back(H.rec_start[size_t(t) + 1] + rec_off);
       all.h1count.push_back(H.h1count[size_t(t)]);
+      all.ext_start.push_back(H.ext_start[size_t(t) + 1] + ext_off);
+      all.nreg.push_back(H.nreg[size_t(t)]);
     }
     for (int32_t g : H.halo_idx) all.halo_idx.push_back(g + node_off);
-    for (TileRec r : H.rec) {
-      if (r.geo & GEO_IRREGULAR) {
-        const uint32_t id = (uint32_t(r.idx[0]) | (uint32_t(r.idx[1]) << 16)) + ext_off;
-        r.idx[0] = uint16_t(id & 0xffffu);
-        r.idx[1] = uint16_t(id >> 16);
-      }
-      all.rec.push_back(r);
-    }
+    all.rec.insert(all.rec.end(), H.rec.begin(), H.rec.end());
     all.rec_slot.insert(all.rec_slot.end(), H.rec_slot.begin(), H.rec_slot.end());
     const uint32_t scale = 1u << (all.lmax - H.lmax);
     for (size_t q = 0; q < H.ext.size(); ++q) {
       uint32_t v = H.ext[q];
-      if (int(q % size_t(W)) >= C) {  // transverse offset words of a face extension
+      const int w = int(q % size_t(WORDS));  // header, then SW words per hanging face
+      if (w >= 1 && w - 1 < NH * SW && (w - 1) % SW >= C) {  // transverse offset words
         v *= scale;
         if (v > 0xffffu) {
           B->tile_error = "extension offset does not fit the batch's finest unit";

@@ -134,6 +134,8 @@ dev::TileArgs<R> tile_args(const casl_stencil *st) {
   ta.rec = reinterpret_cast<const TileRec *>(st->t_rec.p);
   ta.rec_slot = st->t_rec_slot.p;
   ta.ext = st->t_ext.p;
+  ta.ext_start = st->t_ext_start.p;
+  ta.nreg = st->t_nreg.p;
   ta.geom.pow2 = 1;
   for (int a = 0; a < 3; ++a) {
     ta.geom.base[a] = R(st->geom_base[a]);
@@ -330,13 +332,16 @@ namespace casl {
 // anything reaches the GPU: tiles within their LDS limits, halo nodes and
 // extension ids in range, every slot used once, every corner slot readable.
 void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
-  const int F = 2 * D, C = 1 << (D - 1), W = (D == 2 ? 3 : 6);
+  const int F = 2 * D, C = 1 << (D - 1);
+  const int WORDS = D == 2 ? HangExt<2>::WORDS : HangExt<3>::WORDS, SW = D == 2 ? HangExt<2>::SW : HangExt<3>::SW;
   const int LMAX = D == 2 ? TileLimits<2>::LMAX : TileLimits<3>::LMAX;
   const int DMAX = D == 2 ? TileLimits<2>::DMAX : TileLimits<3>::DMAX;
-  const size_t n_ext = H.ext.size() / size_t(F * W);
   const int64_t nt = int64_t(H.tile_start.size()) - 1;
   if (H.tile_start.front() != 0 || H.tile_start.back() != n_nodes)
     throw std::runtime_error("tile ranges do not cover the nodes");
+  if (int64_t(H.ext_start.size()) != nt + 1 || int64_t(H.nreg.size()) != nt || H.ext_start.front() != 0 ||
+      size_t(H.ext_start.back()) * size_t(WORDS) != H.ext.size())
+    throw std::runtime_error("extension ranges do not match the tiles");
   for (int64_t t = 0; t < nt; ++t) {
     const int nT = H.tile_start[t + 1] - H.tile_start[t];
     const int nH = H.halo_start[t + 1] - H.halo_start[t];
@@ -348,6 +353,11 @@ void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
       const int32_t g = H.halo_idx[size_t(H.halo_start[t]) + k];
       if (g < 0 || g >= n_nodes) throw std::runtime_error("halo node out of range");
     }
+    // regular records first within T and within H1; the extension records of
+    // the irregular ones, in record order, fill [ext_start[t], ext_start[t+1])
+    const int nTreg = int(H.nreg[t] & 0xffffu), nH1reg = int(H.nreg[t] >> 16);
+    if (nTreg > nT || nH1reg > nH1 || H.ext_start[t + 1] - H.ext_start[t] != (nT - nTreg) + (nH1 - nH1reg))
+      throw std::runtime_error("regular record counts do not match the extension range");
     std::vector<char> seen(size_t(nR), 0);
     for (int j = 0; j < nR; ++j) {
       const TileRec &rr = H.rec[size_t(H.rec_start[t]) + j];
@@ -356,18 +366,22 @@ void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
       // its H1 nodes (slots nT .. nT + nH1 - 1), each slot exactly once
       if ((j < nT) ? (slot >= nT) : (slot < nT || slot >= nT + nH1)) throw std::runtime_error("record slot out of range");
       if (seen[size_t(slot)]++) throw std::runtime_error("record slot repeated");
+      const bool irregular = (rr.geo & GEO_IRREGULAR) != 0;
+      if (irregular != ((j < nT) ? (j >= nTreg) : (j >= nT + nH1reg)))
+        throw std::runtime_error("record out of its regular/irregular group");
       // pass A of any record reads phi slots < nT + nH; pass B of a tile
       // node also reads second-derivative slots < nT + nH1 at the same corners
       const int bound = (j < nT) ? nT + nH1 : nT + nH;
-      if (rr.geo & GEO_IRREGULAR) {
-        const size_t id = size_t(rr.idx[0]) | (size_t(rr.idx[1]) << 16);
-        if (id >= n_ext) throw std::runtime_error("extension id out of range");
-        for (int f = 0; f < F; ++f)
+      for (int f = 0; f < F; ++f)
+        if (rr.idx[f] >= bound) throw std::runtime_error("corner slot out of range");
+      if (irregular) {
+        const int rank = (j < nT) ? j - nTreg : (nT - nTreg) + (j - nT - nH1reg);
+        const uint16_t *e = &H.ext[(size_t(H.ext_start[t]) + size_t(rank)) * size_t(WORDS)];
+        const int h0 = D == 2 ? e[0] : (e[0] & 15), h1 = D == 2 ? int(HANG_NONE) : ((e[0] >> 4) & 15);
+        if (h0 >= F || (h1 != HANG_NONE && (h1 >= F || h1 <= h0))) throw std::runtime_error("bad hanging face");
+        for (int q = 0; q < (h1 == HANG_NONE ? 1 : 2); ++q)
           for (int c = 0; c < C; ++c)
-            if (H.ext[id * F * W + size_t(f) * W + c] >= bound) throw std::runtime_error("corner slot out of range");
-      } else {
-        for (int f = 0; f < F; ++f)
-          if (rr.idx[f] >= bound) throw std::runtime_error("corner slot out of range");
+            if (e[1 + q * SW + c] >= bound) throw std::runtime_error("corner slot out of range");
       }
     }
   }
@@ -389,7 +403,9 @@ void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st) {
   up(st->t_rec_slot, H.rec_slot);
   st->t_rec.ensure(H.rec.size());
   HIP_CHECK(hipMemcpy(st->t_rec.p, H.rec.data(), H.rec.size() * sizeof(TileRec), hipMemcpyHostToDevice));
-  up(st->t_ext, H.ext.empty() ? std::vector<uint16_t>(8, 0) : H.ext);
+  up(st->t_ext, H.ext.empty() ? std::vector<uint16_t>(16, 0) : H.ext);
+  up(st->t_ext_start, H.ext_start);
+  up(st->t_nreg, H.nreg);
   st->n_tiles = int64_t(H.tile_start.size()) - 1;
   st->n_halo = int64_t(H.halo_idx.size());
   st->n_records = int64_t(H.rec.size());

@@ -210,10 +210,7 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
     ngbd_quadratic<D, R>(t, j, phin, pn_000, pn_f, pn_dd_000);
   }
   R pn_dd_f[F];
-  if constexpr (SECOND_ORDER) {
-#pragma unroll
-    for (int f = 0; f < F; ++f) pn_dd_f[f] = face_linear<D, R>(t, f, j, dd + int64_t(f >> 1) * dd_stride);
-  }
+  if constexpr (SECOND_ORDER) face_values_dd<D, R>(t, j, dd, dd_stride, pn_dd_f);
   R s[F];
 #pragma unroll
   for (int f = 0; f < F; ++f) s[f] = R(t.d(f, j));
@@ -326,11 +323,8 @@ __device__ __forceinline__ double advect_value(const TB &t, int64_t j, uint8_t w
 #pragma unroll
   for (int f = 0; f < F; ++f) s[f] = double(t.d(f, j));
 #pragma unroll
-  for (int a = 0; a < D; ++a) {
-    dd_000[a] = dd[int64_t(a) * dd_stride + j];
-    dd_f[2 * a] = face_linear<D, double>(t, 2 * a, j, dd + int64_t(a) * dd_stride);
-    dd_f[2 * a + 1] = face_linear<D, double>(t, 2 * a + 1, j, dd + int64_t(a) * dd_stride);
-  }
+  for (int a = 0; a < D; ++a) dd_000[a] = dd[int64_t(a) * dd_stride + j];
+  face_values_dd<D, double>(t, j, dd, dd_stride, dd_f);
 #pragma unroll
   for (int a = 0; a < D; ++a) {
     const int fm = 2 * a, fp = 2 * a + 1;
@@ -404,6 +398,8 @@ struct casl_stencil {
   casl::DevBuf<uint16_t> t_rec_slot;
   casl::DevBuf<uint4> t_rec;
   casl::DevBuf<uint16_t> t_ext;
+  casl::DevBuf<int32_t> t_ext_start;
+  casl::DevBuf<uint32_t> t_nreg;
   // casl_reinit_graph: the recorded call and what it was recorded for
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;

@@ -20,6 +20,7 @@ template <int D>
 struct Tables {
   static constexpr bool REGULAR = false;  // see the REGULAR shortcuts below
   static constexpr bool FASTDIV = false;  // see second_difference
+  static constexpr bool SLOTS = false;    // see the hanging-face functions below
   static constexpr int F = 2 * D;
   static constexpr int C = 1 << (D - 1);
   static constexpr int T = 2 * (D - 1);
@@ -218,14 +219,11 @@ __device__ __forceinline__ R second_difference(const TB &t, int a, R fp, R f0, R
 template <class R>
 __device__ __forceinline__ bool check_if_zero(R v) { return fabs(v) < R(EPS); }
 
-// correct_naive_second_derivatives, my_p4est_quad_neighbor_nodes_of_node.cpp:44-298
-template <int D, class R, class TB>
-__device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R &Dyy, R &Dzz) {
-  if constexpr (TB::REGULAR) return;
-  const R m12 = div_dsum<R>(t, div_d<R>(t, R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)), 0, n), 0, n) +
-                div_dsum<R>(t, div_d<R>(t, R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)), 1, n), 0, n);
-  const R m21 = div_dsum<R>(t, div_d<R>(t, R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)), 2, n), 1, n) +
-                div_dsum<R>(t, div_d<R>(t, R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)), 3, n), 1, n);
+// correct_naive_second_derivatives, my_p4est_quad_neighbor_nodes_of_node.cpp:44-298:
+// the solve, given the m_ij (m13 .. m32 unused in 2D)
+template <int D, class R>
+__device__ __forceinline__ void correct_naive_solve(R m12, R m21, R m13, R m23, R m31, R m32, R &Dxx, R &Dyy,
+                                                    R &Dzz) {
   if constexpr (D == 2) {
     if constexpr (sizeof(R) == 4) {  // FP32: the general inverse, as in 3D below
       const R idet = qdiv(R(1.0), R(1.0) - m12 * m21);
@@ -245,14 +243,6 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
       Dyy = nDyy * idet - nDxx * (m21 * idet);
     }
   } else {
-    const R m13 = div_dsum<R>(t, div_d<R>(t, R(t.dt(0, 2, n)) * R(t.dt(0, 3, n)), 0, n), 0, n) +
-                  div_dsum<R>(t, div_d<R>(t, R(t.dt(1, 2, n)) * R(t.dt(1, 3, n)), 1, n), 0, n);
-    const R m23 = div_dsum<R>(t, div_d<R>(t, R(t.dt(2, 2, n)) * R(t.dt(2, 3, n)), 2, n), 1, n) +
-                  div_dsum<R>(t, div_d<R>(t, R(t.dt(3, 2, n)) * R(t.dt(3, 3, n)), 3, n), 1, n);
-    const R m31 = div_dsum<R>(t, div_d<R>(t, R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)), 4, n), 2, n) +
-                  div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)), 5, n), 2, n);
-    const R m32 = div_dsum<R>(t, div_d<R>(t, R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)), 4, n), 2, n) +
-                  div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)), 5, n), 2, n);
     if constexpr (sizeof(R) == 4) {
       // FP32 (the mixed path): the general inverse for every case.  The m_ij
       // are ratios of dyadic lengths, so each is exactly 0 or far above EPS,
@@ -334,9 +324,174 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
   }
 }
 
+// correct_naive_second_derivatives with the m_ij of a stencil's faces
+template <int D, class R, class TB>
+__device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R &Dyy, R &Dzz) {
+  if constexpr (TB::REGULAR) return;
+  const R m12 = div_dsum<R>(t, div_d<R>(t, R(t.dt(0, 0, n)) * R(t.dt(0, 1, n)), 0, n), 0, n) +
+                div_dsum<R>(t, div_d<R>(t, R(t.dt(1, 0, n)) * R(t.dt(1, 1, n)), 1, n), 0, n);
+  const R m21 = div_dsum<R>(t, div_d<R>(t, R(t.dt(2, 0, n)) * R(t.dt(2, 1, n)), 2, n), 1, n) +
+                div_dsum<R>(t, div_d<R>(t, R(t.dt(3, 0, n)) * R(t.dt(3, 1, n)), 3, n), 1, n);
+  if constexpr (D == 2) {
+    correct_naive_solve<D, R>(m12, m21, R(0), R(0), R(0), R(0), Dxx, Dyy, Dzz);
+  } else {
+    const R m13 = div_dsum<R>(t, div_d<R>(t, R(t.dt(0, 2, n)) * R(t.dt(0, 3, n)), 0, n), 0, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(1, 2, n)) * R(t.dt(1, 3, n)), 1, n), 0, n);
+    const R m23 = div_dsum<R>(t, div_d<R>(t, R(t.dt(2, 2, n)) * R(t.dt(2, 3, n)), 2, n), 1, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(3, 2, n)) * R(t.dt(3, 3, n)), 3, n), 1, n);
+    const R m31 = div_dsum<R>(t, div_d<R>(t, R(t.dt(4, 0, n)) * R(t.dt(4, 1, n)), 4, n), 2, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 0, n)) * R(t.dt(5, 1, n)), 5, n), 2, n);
+    const R m32 = div_dsum<R>(t, div_d<R>(t, R(t.dt(4, 2, n)) * R(t.dt(4, 3, n)), 4, n), 2, n) +
+                  div_dsum<R>(t, div_d<R>(t, R(t.dt(5, 2, n)) * R(t.dt(5, 3, n)), 5, n), 2, n);
+    correct_naive_solve<D, R>(m12, m21, m13, m23, m31, m32, Dxx, Dyy, Dzz);
+  }
+}
+
+// ------------------------------------------------- hanging faces by slot
+// A stencil with TB::SLOTS (a node of the tiled encoding with hanging arms)
+// names the faces that hang: t.hface(q) for q < TB::NH (HANG_NONE past the
+// last one), with corners t.hidx(q, c) and transverse offsets
+// t.hdt(q, k, q_f) (k: m0, p0, 0m, 0p; q_f the arm cell's size).  Each other
+// face f is regular: one corner t.id[f] carries its value and its transverse
+// products d_m0 d_p0, d_0m d_0p are 0, so it adds nothing to the m_ij of
+// correct_naive_second_derivatives nor to the quadratic corrections.  The
+// functions below do the hanging-face work for the hanging faces only; the
+// terms they leave out are exact zeros (finite inputs), so their results are
+// those of the per-face functions above, in FP64 bit for bit.
+
+// interpolate_linearly on hanging face q (my_p4est_quad_neighbor_nodes_of_node.h:222-306).
+// The transverse sums are the arm cell's size q_f (cells are cubes in the
+// tiled encoding), so on a dyadic geometry the division is a multiplication
+// by the exact inverse iqf (squared in 3D).
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ R hang_face_linear(const TB &t, int q, R qf, R iqf, const G *__restrict__ g) {
+  const R d_m0 = t.hdt(q, 0, qf), d_p0 = t.hdt(q, 1, qf);
+  if constexpr (D == 2) {
+    const R num = qfma(R(g[t.hidx(q, 0)]), d_p0, R(g[t.hidx(q, 1)]) * d_m0);
+    if (t.pow2) return num * iqf;
+    return qdiv(num, d_m0 + d_p0);
+  } else {
+    const R d_0m = t.hdt(q, 2, qf), d_0p = t.hdt(q, 3, qf);
+    const R w_mm = d_p0 * d_0p, w_pm = d_m0 * d_0p, w_mp = d_p0 * d_0m, w_pp = d_m0 * d_0m;
+    const R num = qfma(R(g[t.hidx(q, 0)]), w_mm,
+                       qfma(R(g[t.hidx(q, 2)]), w_mp, qfma(R(g[t.hidx(q, 1)]), w_pm, R(g[t.hidx(q, 3)]) * w_pp)));
+    if (t.pow2) return num * (iqf * iqf);
+    return qdiv(num, (d_m0 + d_p0) * (d_0m + d_0p));
+  }
+}
+
+// the quadratic correction of hanging face q (axis a), as quadratic_corrections
+template <int D, class R, class TB>
+__device__ __forceinline__ R hang_correction(const TB &t, int q, int a, R qf, const R *dd) {
+  const R h = R(0.5);
+  if constexpr (D == 2) {
+    return h * t.hdt(q, 0, qf) * t.hdt(q, 1, qf) * (a == 0 ? dd[1] : dd[0]);
+  } else {
+    const R dd1 = (a == 0) ? dd[1] : dd[0], dd2 = (a == 2) ? dd[1] : dd[2];
+    return qfma(h * t.hdt(q, 0, qf) * t.hdt(q, 1, qf), dd1, h * t.hdt(q, 2, qf) * t.hdt(q, 3, qf) * dd2);
+  }
+}
+
+// every face's value of g: its corner's on regular faces, the interpolation
+// (minus the quadratic correction with the node's dd, CORR) on hanging ones;
+// vq[q] keeps hanging face q's value
+template <int D, class R, bool CORR, class G, class TB>
+__device__ __forceinline__ void hang_face_values(const TB &t, const G *__restrict__ g, R *fnb, const R *dd, R *vq) {
+  constexpr int F = 2 * D;
+#pragma unroll
+  for (int f = 0; f < F; ++f) fnb[f] = R(g[t.id[f]]);
+#pragma unroll
+  for (int q = 0; q < TB::NH; ++q) {
+    const int f = t.hface(q);
+    if (q == 0 || f < F) {
+      const auto sg = t.slot_geom(f);
+      R v = hang_face_linear<D, R>(t, q, sg.qf, sg.iqf, g);
+      if constexpr (CORR) v -= hang_correction<D, R>(t, q, f >> 1, sg.qf, dd);
+      vq[q] = v;
+#pragma unroll
+      for (int i = 0; i < F; ++i) fnb[i] = (i == f) ? v : fnb[i];
+    }
+  }
+}
+
+// per face f, the value of component f >> 1 of dd (entries stride apart)
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ void hang_face_values_dd(const TB &t, const G *__restrict__ dd, int64_t stride, R *out) {
+  constexpr int F = 2 * D;
+#pragma unroll
+  for (int f = 0; f < F; ++f) out[f] = R(dd[int64_t(f >> 1) * stride + t.id[f]]);
+#pragma unroll
+  for (int q = 0; q < TB::NH; ++q) {
+    const int f = t.hface(q);
+    if (q == 0 || f < F) {
+      const auto sg = t.slot_geom(f);
+      const R v = hang_face_linear<D, R>(t, q, sg.qf, sg.iqf, dd + int64_t(f >> 1) * stride);
+#pragma unroll
+      for (int i = 0; i < F; ++i) out[i] = (i == f) ? v : out[i];
+    }
+  }
+}
+
+// correct_naive_second_derivatives from the hanging faces' m terms: face f
+// of axis a adds d_m0 d_p0 / d_f / (d_a+ + d_a-) to m_a,first transverse and
+// (3D) d_0m d_0p / d_f / (d_a+ + d_a-) to m_a,second, as div_d / div_dsum
+template <int D, class R, class TB>
+__device__ __forceinline__ void hang_correct_naive(const TB &t, R &Dxx, R &Dyy, R &Dzz) {
+  constexpr int F = 2 * D;
+  R m[3][2] = {{R(0), R(0)}, {R(0), R(0)}, {R(0), R(0)}};
+#pragma unroll
+  for (int q = 0; q < TB::NH; ++q) {
+    const int f = t.hface(q);
+    if (q == 0 || f < F) {
+      const int a = f >> 1;
+      const auto sg = t.slot_geom(f);
+      const R qf = sg.qf, df = sg.df, iqf = sg.iqf, dp = sg.dp, dm = sg.dm, iqp = sg.iqp;
+      auto term = [&](R x) {
+        // div_d(x, f), then div_dsum(., a)
+        if constexpr (irregular_fastdiv<R, TB>()) {
+          if (t.fast) x = x * iqf;
+          else x = qdiv(x, df);
+          if (t.fast && dp == dm) return x * (R(0.5) * iqp);
+          return qdiv(x, dp + dm);
+        } else {
+          return qdiv(qdiv(x, df), dp + dm);
+        }
+      };
+      const R T1 = term(t.hdt(q, 0, qf) * t.hdt(q, 1, qf));
+#pragma unroll
+      for (int b = 0; b < D; ++b) m[b][0] = m[b][0] + ((a == b) ? T1 : R(0));
+      if constexpr (D == 3) {
+        const R T2 = term(t.hdt(q, 2, qf) * t.hdt(q, 3, qf));
+#pragma unroll
+        for (int b = 0; b < D; ++b) m[b][1] = m[b][1] + ((a == b) ? T2 : R(0));
+      }
+    }
+  }
+  correct_naive_solve<D, R>(m[0][0], m[1][0], m[0][1], m[1][1], m[2][0], m[2][1], Dxx, Dyy, Dzz);
+}
+
+// laplace of a node with hanging faces; vq[q] keeps hanging face q's value
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ void hang_laplace(const TB &t, int64_t n, const G *__restrict__ g, R &f000, R *fnb, R *dd,
+                                             R *vq) {
+  f000 = R(g[n]);
+  hang_face_values<D, R, false>(t, g, fnb, dd, vq);
+  dd[0] = second_difference<R>(t, 0, fnb[1], f000, fnb[0], n);
+  dd[1] = second_difference<R>(t, 1, fnb[3], f000, fnb[2], n);
+  R dzz = R(0);
+  if constexpr (D == 3) dzz = second_difference<R>(t, 2, fnb[5], f000, fnb[4], n);
+  hang_correct_naive<D, R>(t, dd[0], dd[1], dzz);
+  if constexpr (D == 3) dd[2] = dzz;
+}
+
 // laplace, my_p4est_quad_neighbor_nodes_of_node.h:615-635 (one field)
 template <int D, class R, class G, class TB>
 __device__ __forceinline__ void laplace(const TB &t, int64_t n, const G *__restrict__ g, R &f000, R *fnb, R *dd) {
+  if constexpr (TB::SLOTS) {
+    R vq[TB::NH];
+    hang_laplace<D, R>(t, n, g, f000, fnb, dd, vq);
+    return;
+  }
   f000 = R(g[n]);
 #pragma unroll
   for (int f = 0; f < 2 * D; ++f) fnb[f] = face_linear<D, R>(t, f, n, g);
@@ -373,8 +528,23 @@ __device__ __forceinline__ void quadratic_corrections(const TB &t, int64_t n, R 
 template <int D, class R, class G, class TB>
 __device__ __forceinline__ void ngbd_quadratic(const TB &t, int64_t n, const G *__restrict__ g, R &f000, R *fnb,
                                                R *dd) {
-  laplace<D, R>(t, n, g, f000, fnb, dd);
-  quadratic_corrections<D, R>(t, n, fnb, dd);
+  if constexpr (TB::SLOTS) {
+    constexpr int F = 2 * D;
+    R vq[TB::NH];
+    hang_laplace<D, R>(t, n, g, f000, fnb, dd, vq);
+#pragma unroll
+    for (int q = 0; q < TB::NH; ++q) {
+      const int f = t.hface(q);
+      if (q == 0 || f < F) {
+        const R v = vq[q] - hang_correction<D, R>(t, q, f >> 1, t.slot_geom(f).qf, dd);
+#pragma unroll
+        for (int i = 0; i < F; ++i) fnb[i] = (i == f) ? v : fnb[i];
+      }
+    }
+  } else {
+    laplace<D, R>(t, n, g, f000, fnb, dd);
+    quadratic_corrections<D, R>(t, n, fnb, dd);
+  }
 }
 
 // The same with the node's own second derivatives already known (they are the
@@ -383,9 +553,27 @@ template <int D, class R, class G, class TB>
 __device__ __forceinline__ void ngbd_quadratic_given_dd(const TB &t, int64_t n, const G *__restrict__ g,
                                                         const R *dd, R &f000, R *fnb) {
   f000 = R(g[n]);
+  if constexpr (TB::SLOTS) {
+    R vq[TB::NH];
+    hang_face_values<D, R, true>(t, g, fnb, dd, vq);
+    return;
+  }
 #pragma unroll
   for (int f = 0; f < 2 * D; ++f) fnb[f] = face_linear<D, R>(t, f, n, g);
   quadratic_corrections<D, R>(t, n, fnb, dd);
+}
+
+// per face f, the linear face value of component f >> 1 of dd (entries
+// stride apart): f_{m00,p00,...}_linear of the stored second derivatives
+template <int D, class R, class G, class TB>
+__device__ __forceinline__ void face_values_dd(const TB &t, int64_t n, const G *__restrict__ dd, int64_t stride,
+                                               R *out) {
+  if constexpr (TB::SLOTS) {
+    hang_face_values_dd<D, R>(t, dd, stride, out);
+  } else {
+#pragma unroll
+    for (int f = 0; f < 2 * D; ++f) out[f] = face_linear<D, R>(t, f, n, dd + int64_t(f >> 1) * stride);
+  }
 }
 
 // ---------------------------------------------------- first derivatives

@@ -118,8 +118,10 @@ struct Builder {
     std::vector<int32_t> h1, h2;
     std::vector<TileRec> rec;
     std::vector<uint16_t> slot;
-    std::vector<uint16_t> ext;  // extensions of this tile's irregular records
+    std::vector<uint16_t> ext;  // extension records of this tile's irregular records, in record order
+    int n_treg = 0, n_h1reg = 0;  // regular records of T and of H1
   };
+  static constexpr int NH = HangExt<D>::NH, WORDS = HangExt<D>::WORDS, SW = HangExt<D>::SW;
 
   // local index of global node g in tile (s, e, h1, h2)
   static int32_t local(const Tile &t, int32_t g) {
@@ -159,37 +161,39 @@ struct Builder {
     return nT + int64_t(t.h1.size()) + int64_t(t.h2.size()) <= TileLimits<D>::LMAX;
   }
 
-  TileRec encode(const Tile &t, int64_t n, std::vector<uint16_t> &ext) const {
+  // The record of node n; for a node with hanging faces also its extension
+  // record (WORDS words at e) and, in *nh, how many faces hang (0: regular).
+  TileRec encode(const Tile &t, int64_t n, uint16_t *e, int *nh) const {
     TileRec r{};
     uint32_t geo = 0;
-    bool irregular = false;
+    int hf[2] = {HANG_NONE, HANG_NONE}, k = 0;
     for (int f = 0; f < F; ++f) {
       geo |= uint32_t(level_of(f >> 1, arm_cell(f, n))) << (4 * f);
       if (wall(f, n)) geo |= 1u << (24 + f);
       double w[4];
       weights(f, n, w);
-      int nz = 0;
-      for (int c = 0; c < C; ++c) nz += (w[c] != 0.0);
-      if (nz != 1) irregular = true;
-    }
-    if (!irregular) {
-      for (int f = 0; f < F; ++f) {
-        double w[4];
-        weights(f, n, w);
-        for (int c = 0; c < C; ++c)
-          if (w[c] != 0.0) {
-            const int32_t l = local(t, nbr(f, c, n));
-            if (l < 0 || l > 65535) throw std::runtime_error("tile-local index out of range");
-            r.idx[f] = uint16_t(l);
-          }
+      int nz = 0, first = -1;
+      for (int c = 0; c < C; ++c)
+        if (w[c] != 0.0) {
+          ++nz;
+          if (first < 0) first = c;
+        }
+      if (first < 0) throw std::runtime_error("face without a nonzero interpolation weight");
+      const int32_t l = local(t, nbr(f, first, n));
+      if (l < 0 || l > 65535) throw std::runtime_error("tile-local index out of range");
+      r.idx[f] = uint16_t(l);
+      if (nz != 1) {
+        if (k == NH) throw std::runtime_error("more hanging faces than the extension record holds");
+        hf[k++] = f;
       }
-    } else {
+    }
+    *nh = k;
+    if (k > 0) {
       geo |= GEO_IRREGULAR;
-      const size_t id = ext.size() / (F * (D == 2 ? 3 : 6));
-      if (id > 0xFFFFFFFFull) throw std::runtime_error("too many irregular nodes");
-      r.idx[0] = uint16_t(id & 0xFFFF);
-      r.idx[1] = uint16_t(id >> 16);
-      for (int f = 0; f < F; ++f) {
+      std::fill(e, e + WORDS, uint16_t(0));
+      e[0] = uint16_t(D == 2 ? hf[0] : (hf[0] | (hf[1] << 4)));
+      for (int q = 0; q < k; ++q) {
+        const int f = hf[q];
         int t1, t2;
         transverse(f >> 1, t1, t2);
         double w[4];
@@ -197,22 +201,24 @@ struct Builder {
         int first = -1;
         for (int c = 0; c < C; ++c)
           if (w[c] != 0.0 && first < 0) first = c;
+        uint16_t *fe = e + 1 + q * SW;
         for (int c = 0; c < C; ++c) {
           const int32_t l = local(t, nbr(f, w[c] != 0.0 ? c : first, n));
           if (l < 0 || l > 65535) throw std::runtime_error("tile-local index out of range");
-          ext.push_back(uint16_t(l));
+          fe[c] = uint16_t(l);
         }
-        ext.push_back(units(t1, dt(f, 0, n)));
-        if (D == 3) ext.push_back(units(t2, dt(f, 2, n)));
+        fe[C] = units(t1, dt(f, 0, n));
+        if (D == 3) fe[C + 1] = units(t2, dt(f, 2, n));
       }
     }
     r.geo = geo;
-    verify(t, n, r, ext);
+    verify(t, n, r, e);
     return r;
   }
 
-  // decode == reference stencil (distances bit-exact, nonzero-weight corners equal)
-  void verify(const Tile &t, int64_t n, const TileRec &r, const std::vector<uint16_t> &ext) const {
+  // decode == reference stencil (distances bit-exact, nonzero-weight corners
+  // equal); e is the extension record of an irregular record
+  void verify(const Tile &t, int64_t n, const TileRec &r, const uint16_t *e) const {
     double dd[6];
     for (int f = 0; f < F; ++f) {
       const int lv = (r.geo >> (4 * f)) & 15;
@@ -225,30 +231,40 @@ struct Builder {
       }
       if (dd[f] != d(f, n)) throw std::runtime_error("tile record: face distance mismatch");
     }
+    int hf[2] = {HANG_NONE, HANG_NONE};
     if (r.geo & GEO_IRREGULAR) {
-      const size_t id = size_t(r.idx[0]) | (size_t(r.idx[1]) << 16);
-      const uint16_t *e = &ext[id * F * (D == 2 ? 3 : 6)];
-      for (int f = 0; f < F; ++f) {
-        int t1, t2;
-        transverse(f >> 1, t1, t2);
-        const uint16_t *fe = e + f * (D == 2 ? 3 : 6);
-        const double qh1 = base[t1] * std::ldexp(1.0, -int((r.geo >> (4 * f)) & 15));
-        const double m0 = fe[C] * unit[t1];
-        const double p0 = qh1 - m0;
-        if (m0 != dt(f, 0, n) || p0 != dt(f, 1, n)) throw std::runtime_error("tile record: offset mismatch");
-        if (D == 3) {
-          const double qh2 = base[t2] * std::ldexp(1.0, -int((r.geo >> (4 * f)) & 15));
-          const double zm = fe[C + 1] * unit[t2];
-          if (zm != dt(f, 2, n) || qh2 - zm != dt(f, 3, n)) throw std::runtime_error("tile record: offset mismatch");
-        }
-      }
-    } else {
-      for (int f = 0; f < F; ++f) {
-        double w[4];
-        weights(f, n, w);
+      hf[0] = D == 2 ? e[0] : (e[0] & 15);
+      if (D == 3) hf[1] = (e[0] >> 4) & 15;
+      if (hf[0] >= F || (hf[1] != HANG_NONE && (hf[1] >= F || hf[1] <= hf[0])))
+        throw std::runtime_error("tile record: bad hanging face");
+    }
+    for (int f = 0; f < F; ++f) {
+      double w[4];
+      weights(f, n, w);
+      const int q = (f == hf[0]) ? 0 : (f == hf[1]) ? 1 : -1;
+      if (q < 0) {
+        // one corner carries the face value: the record's
+        int nz = 0;
         for (int c = 0; c < C; ++c)
-          if (w[c] != 0.0 && local(t, nbr(f, c, n)) != r.idx[f])
-            throw std::runtime_error("tile record: corner mismatch");
+          if (w[c] != 0.0) {
+            ++nz;
+            if (local(t, nbr(f, c, n)) != r.idx[f]) throw std::runtime_error("tile record: corner mismatch");
+          }
+        if (nz != 1) throw std::runtime_error("tile record: hanging face not in the extension");
+        continue;
+      }
+      int t1, t2;
+      transverse(f >> 1, t1, t2);
+      const uint16_t *fe = e + 1 + q * SW;
+      for (int c = 0; c < C; ++c)
+        if (w[c] != 0.0 && local(t, nbr(f, c, n)) != fe[c]) throw std::runtime_error("tile record: corner mismatch");
+      const double qh1 = base[t1] * std::ldexp(1.0, -int((r.geo >> (4 * f)) & 15));
+      const double m0 = fe[C] * unit[t1];
+      if (m0 != dt(f, 0, n) || qh1 - m0 != dt(f, 1, n)) throw std::runtime_error("tile record: offset mismatch");
+      if (D == 3) {
+        const double qh2 = base[t2] * std::ldexp(1.0, -int((r.geo >> (4 * f)) & 15));
+        const double zm = fe[C + 1] * unit[t2];
+        if (zm != dt(f, 2, n) || qh2 - zm != dt(f, 3, n)) throw std::runtime_error("tile record: offset mismatch");
       }
     }
   }
@@ -258,22 +274,35 @@ struct Builder {
     t.rec.clear();
     t.slot.clear();
     t.ext.clear();
-    std::vector<TileRec> all;
-    all.reserve(nT + t.h1.size());
-    for (int64_t n = t.s; n < t.e; ++n) all.push_back(encode(t, n, t.ext));
-    for (int32_t h : t.h1) all.push_back(encode(t, h, t.ext));
-    // group: T regular, T hanging, H1 regular, H1 hanging
-    auto emit = [&](size_t lo, size_t hi, bool irregular) {
+    const size_t nall = size_t(nT) + t.h1.size();
+    std::vector<TileRec> all(nall);
+    std::vector<uint16_t> exts(nall * WORDS);
+    std::vector<int> kind(nall);  // number of hanging faces
+    for (int64_t n = t.s; n < t.e; ++n) {
+      const size_t j = size_t(n - t.s);
+      all[j] = encode(t, n, &exts[j * WORDS], &kind[j]);
+    }
+    for (size_t i = 0; i < t.h1.size(); ++i) {
+      const size_t j = size_t(nT) + i;
+      all[j] = encode(t, t.h1[i], &exts[j * WORDS], &kind[j]);
+    }
+    // group: T regular, T hanging on 1 .. NH faces, then the same for H1;
+    // extension records follow the irregular records in that order
+    auto emit = [&](size_t lo, size_t hi, int k) {
+      int cnt = 0;
       for (size_t j = lo; j < hi; ++j)
-        if (bool(all[j].geo & GEO_IRREGULAR) == irregular) {
+        if (kind[j] == k) {
           t.rec.push_back(all[j]);
           t.slot.push_back(uint16_t(j));
+          if (k > 0) t.ext.insert(t.ext.end(), exts.begin() + j * WORDS, exts.begin() + (j + 1) * WORDS);
+          ++cnt;
         }
+      return cnt;
     };
-    emit(0, size_t(nT), false);
-    emit(0, size_t(nT), true);
-    emit(size_t(nT), all.size(), false);
-    emit(size_t(nT), all.size(), true);
+    t.n_treg = emit(0, size_t(nT), 0);
+    for (int k = 1; k <= NH; ++k) emit(0, size_t(nT), k);
+    t.n_h1reg = emit(size_t(nT), nall, 0);
+    for (int k = 1; k <= NH; ++k) emit(size_t(nT), nall, k);
   }
 
   // splits [s, e) until every piece fits the LDS limits
@@ -329,7 +358,8 @@ struct Builder {
     H.halo_idx.clear();
     H.rec.clear();
     H.ext.clear();
-    const int ext_words = F * (D == 2 ? 3 : 6);
+    H.ext_start.assign(1, 0);
+    H.nreg.clear();
     size_t n_ext = 0;
     for (auto &p : parts)
       for (auto &t : p) {
@@ -338,21 +368,14 @@ struct Builder {
         H.halo_idx.insert(H.halo_idx.end(), t.h1.begin(), t.h1.end());
         H.halo_idx.insert(H.halo_idx.end(), t.h2.begin(), t.h2.end());
         H.halo_start.push_back(int32_t(H.halo_idx.size()));
-        // rebase extension ids of this tile
-        const size_t base_id = n_ext;
-        for (auto &r : t.rec) {
-          if (r.geo & GEO_IRREGULAR) {
-            const size_t id = (size_t(r.idx[0]) | (size_t(r.idx[1]) << 16)) + base_id;
-            if (id > 0xFFFFFFFFull) throw std::runtime_error("too many irregular records");
-            r.idx[0] = uint16_t(id & 0xFFFF);
-            r.idx[1] = uint16_t(id >> 16);
-          }
-          H.rec.push_back(r);
-        }
+        H.rec.insert(H.rec.end(), t.rec.begin(), t.rec.end());
         H.rec_slot.insert(H.rec_slot.end(), t.slot.begin(), t.slot.end());
         H.rec_start.push_back(int32_t(H.rec.size()));
         H.ext.insert(H.ext.end(), t.ext.begin(), t.ext.end());
-        n_ext += t.ext.size() / ext_words;
+        n_ext += t.ext.size() / WORDS;
+        if (n_ext >= size_t(INT32_MAX)) throw std::runtime_error("too many irregular records");
+        H.ext_start.push_back(int32_t(n_ext));
+        H.nreg.push_back(uint32_t(t.n_treg) | (uint32_t(t.n_h1reg) << 16));
       }
     H.n_irregular = int64_t(n_ext);
     if (H.halo_idx.size() >= size_t(INT32_MAX) || H.rec.size() >= size_t(INT32_MAX))

@@ -10,12 +10,16 @@
 //   idx[f]     LDS-local index of the one corner with nonzero weight on
 //              face f (a "regular" node: no arm hangs on a coarser cell)
 //   geo        4-bit level of each face's arm cell, wall bits, irregular bit
-// A node with a hanging arm ("irregular") stores a 32-bit index into an
-// extension table instead, whose per-face entry holds all corner indices and
-// the transverse offsets (in units of the finest cell of the tree).
-// Within a tile the records are grouped (T regular, T hanging, H1 regular,
-// H1 hanging) so wavefronts rarely mix the two code paths; rec_slot gives
-// each record's LDS slot.
+// A node with a hanging arm ("irregular") keeps that record for its faces
+// that do not hang and has an extension record (HangExt) for the faces that
+// do: at most D - 1 of them, each with all its corner indices and transverse
+// offsets (in units of the finest cell of the tree).  Extension records are
+// not addressed from the record: within a tile the records are grouped (T
+// regular, T hanging on one face, T hanging on two faces, then the same for
+// H1), the extension records of a tile follow its irregular records in that
+// order from ext_start[tile], and nreg[tile] counts the regular records of T
+// and of H1, which gives each irregular record its rank.  The grouping also
+// keeps wavefronts on one code path; rec_slot gives each record's LDS slot.
 // Decoding reproduces the flattened stencil of the node exactly (zero-weight
 // corners aside, which contribute exactly 0 for finite values); the builder
 // verifies this for every record.
@@ -36,19 +40,25 @@ static_assert(sizeof(TileRec) == 16, "TileRec must be 16 bytes");
 
 constexpr uint32_t GEO_IRREGULAR = 1u << 30;
 
+// Extension record layout, in uint16 words:
+//   2D (4 words):  face, c0, c1, m0
+//   3D (16 words): face0 | face1 << 4 (face1 = HANG_NONE for one hanging
+//                  face; face0 < face1), then per face c0 c1 c2 c3 m0 zm,
+//                  then 3 words of padding
+// c*: tile-local corner indices in the reference's corner order; m0: d_m0
+// along the first transverse axis, zm: d_0m along the second, in units of
+// the finest cell.
 template <int D>
-struct FaceExt;
+struct HangExt;
 template <>
-struct FaceExt<2> {
-  uint16_t c[2];
-  uint16_t m0;  // d_m0 in units of the finest cell along the transverse axis
+struct HangExt<2> {
+  static constexpr int NH = 1, WORDS = 4, SW = 3;  // faces, words, words per face
 };
 template <>
-struct FaceExt<3> {
-  uint16_t c[4];
-  uint16_t m0;  // along the first transverse axis
-  uint16_t zm;  // d_0m along the second transverse axis
+struct HangExt<3> {
+  static constexpr int NH = 2, WORDS = 16, SW = 6;
 };
+constexpr uint16_t HANG_NONE = 15;
 
 // Tile sizing: at most TMAX tile nodes, LMAX nodes in T u H1 u H2 (LDS phi
 // slots) and DMAX nodes in T u H1 (LDS second-derivative slots).  The
@@ -81,7 +91,9 @@ struct TileHost {
   std::vector<int32_t> rec_start;   // [ntiles + 1] ranges in rec (T u H1 of each tile)
   std::vector<TileRec> rec;         // per tile: T regular, T hanging, H1 regular, H1 hanging
   std::vector<uint16_t> rec_slot;   // LDS slot (local index) of each record
-  std::vector<uint16_t> ext;        // irregular extensions, FaceExt<D>[2D] each, as raw uint16
+  std::vector<uint16_t> ext;        // HangExt<D> records, as raw uint16
+  std::vector<int32_t> ext_start;   // [ntiles + 1] first extension record of each tile
+  std::vector<uint32_t> nreg;       // [ntiles] regular records of T (low 16 bits) and of H1 (high 16 bits)
   int64_t n_irregular = 0;
 };
 

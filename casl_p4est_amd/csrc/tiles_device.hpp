@@ -74,6 +74,7 @@ __device__ __forceinline__ void decode_dist(uint32_t geo, const TileGeom<R> &g, 
 template <int D, class R>
 struct DecodedDist {
   static constexpr bool FASTDIV = true;
+  static constexpr bool SLOTS = false;
   static constexpr int F = 2 * D;
   R dist[F];
   R q[F];
@@ -102,26 +103,83 @@ struct RegStencil : DecodedDist<D, R> {
   }
 };
 
-// Stencil of a node with a hanging arm: corners and transverse offsets are
-// read from its extension record when the math asks for them.
+// Stencil of a node with hanging arms: the record's one corner per face, and
+// for the faces that hang (at most D - 1) the extension record's corners and
+// transverse offsets.  idx / dt serve any face (the per-face functions); the
+// hanging-face functions of stencil_math.hpp use hface / hidx / hdt.
 template <int D, class R>
-struct IrrStencil : DecodedDist<D, R> {
+struct HangStencil : DecodedDist<D, R> {
   static constexpr bool REGULAR = false;
+  static constexpr bool SLOTS = true;
   static constexpr int F = 2 * D, C = 1 << (D - 1);
-  static constexpr int W = (D == 2 ? 3 : 6);  // uint16 words per face extension
-  static constexpr int NW = F * W / 2;        // 32-bit words per record (8-byte aligned)
-  uint32_t e[NW];                             // the record, loaded once with 8-byte loads
+  static constexpr int NH = HangExt<D>::NH, WORDS = HangExt<D>::WORDS, SW = HangExt<D>::SW;
+  static constexpr int NW = WORDS / 2;  // 32-bit words (the record is whole 8-byte words)
+  uint16_t id[F];
+  uint32_t e[NW];
   R unit;
-  __device__ __forceinline__ uint32_t word(int i) const { return (e[i >> 1] >> (16 * (i & 1))) & 0xffffu; }
-  __device__ __forceinline__ int32_t idx(int f, int c, int64_t) const { return int32_t(word(f * W + c)); }
-  __device__ __forceinline__ R dt(int f, int k, int64_t) const {
-    const R o = R(word(f * W + C + (k >> 1))) * unit;  // m0 (k = 0, 1) or 0m (k = 2, 3)
-    return (k & 1) ? this->q[f] - o : o;
+  bool pow2;
+  uint32_t geo;
+  R base[D], inv_base[D];  // the geometry's (uniform values)
+  // The arm cell size, its inverse and the face distances of the axis of a
+  // face f known only at run time, from the level nibbles and wall bits as
+  // decode_dist computes them (no indexing of the per-face arrays, which
+  // would move them to scratch memory).
+  struct SlotGeom {
+    R qf, iqf;  // arm cell of face f and its inverse
+    R df;       // face distance of f
+    R dp, dm;   // face distances of the + and - faces of f's axis
+    R iqp;      // inverse arm cell of the + face
+  };
+  __device__ __forceinline__ SlotGeom slot_geom(int f) const {
+    const int a = f >> 1;
+    R b = base[0], ib = inv_base[0];
+#pragma unroll
+    for (int k = 1; k < D; ++k) {
+      b = (a == k) ? base[k] : b;
+      ib = (a == k) ? inv_base[k] : ib;
+    }
+    const int le = int((geo >> (8 * a)) & 15u), lo = int((geo >> (8 * a + 4)) & 15u);
+    const R qe = b * pow2neg(R(0), le), qo = b * pow2neg(R(0), lo);
+    const R de = ((geo >> (24 + 2 * a)) & 1u) ? -qo - qe : qe;
+    const R dO = ((geo >> (25 + 2 * a)) & 1u) ? -de - qo : qo;
+    SlotGeom r;
+    r.qf = (f & 1) ? qo : qe;
+    r.iqf = ib * pow2pos(R(0), (f & 1) ? lo : le);
+    r.df = (f & 1) ? dO : de;
+    r.dp = dO;
+    r.dm = de;
+    r.iqp = ib * pow2pos(R(0), lo);
+    return r;
   }
-  __device__ __forceinline__ void load(const TileRec &r, const uint16_t *__restrict__ ext, const TileGeom<R> &g) {
-    static_assert(NW % 2 == 0, "ext records are whole 8-byte words");
-    const uint32_t id = uint32_t(r.idx[0]) | (uint32_t(r.idx[1]) << 16);
-    const uint2 *src = reinterpret_cast<const uint2 *>(ext + size_t(id) * (F * W));
+  __device__ __forceinline__ uint32_t word(int i) const { return (e[i >> 1] >> (16 * (i & 1))) & 0xffffu; }
+  // face of hanging slot q (HANG_NONE when the node has fewer)
+  __device__ __forceinline__ int hface(int q) const {
+    return D == 2 ? int(word(0)) : int((word(0) >> (4 * q)) & 15u);
+  }
+  __device__ __forceinline__ int32_t hidx(int q, int c) const { return int32_t(word(1 + q * SW + c)); }
+  // transverse offset k (m0, p0, 0m, 0p) of slot q, qf its arm cell's size
+  __device__ __forceinline__ R hdt(int q, int k, R qf) const {
+    const R o = R(word(1 + q * SW + C + (k >> 1))) * unit;
+    return (k & 1) ? qf - o : o;
+  }
+  __device__ __forceinline__ int32_t idx(int f, int c, int64_t) const {
+    int32_t r = id[f];
+#pragma unroll
+    for (int q = 0; q < NH; ++q) r = (hface(q) == f) ? hidx(q, c) : r;
+    return r;
+  }
+  // a regular face's offsets (0, q_f, 0, q_f) weight its one corner only
+  __device__ __forceinline__ R dt(int f, int k, int64_t) const {
+    R r = (k & 1) ? this->q[f] : R(0);
+#pragma unroll
+    for (int q = 0; q < NH; ++q) r = (hface(q) == f) ? hdt(q, k, this->q[f]) : r;
+    return r;
+  }
+  // rec: the node's record; xrec: its extension record (WORDS uint16)
+  __device__ __forceinline__ void load(const TileRec &rec, const uint16_t *__restrict__ xrec, const TileGeom<R> &g) {
+#pragma unroll
+    for (int f = 0; f < F; ++f) id[f] = rec.idx[f];
+    const uint2 *src = reinterpret_cast<const uint2 *>(xrec);
 #pragma unroll
     for (int i = 0; i < NW / 2; ++i) {
       const uint2 v = src[i];
@@ -129,9 +187,26 @@ struct IrrStencil : DecodedDist<D, R> {
       e[2 * i + 1] = v.y;
     }
     unit = g.unit[0];
-    this->decode(r.geo, g);
+    pow2 = g.pow2 != 0;
+    geo = rec.geo;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      base[k] = g.base[k];
+      inv_base[k] = g.inv_base[k];
+    }
+    this->decode(rec.geo, g);
   }
 };
+
+// Extension record of the irregular record at tile-local position i of a
+// tile whose extension records start at ext0 (nreg: regular records of T in
+// the low 16 bits, of H1 in the high 16 bits; nT tile nodes).
+__device__ __forceinline__ const uint16_t *hang_record(const uint16_t *__restrict__ ext, int ext0, uint32_t nreg,
+                                                      int nT, int i, int words) {
+  const int nTreg = int(nreg & 0xffffu), nH1reg = int(nreg >> 16);
+  const int rank = (i < nT) ? i - nTreg : (nT - nTreg) + (i - nT - nH1reg);
+  return ext + (int64_t(ext0) + rank) * words;
+}
 
 // Blocks are dispatched round-robin over the 8 XCDs; give every XCD one
 // contiguous run of tiles so neighbouring tiles (which share halo nodes)
@@ -153,6 +228,8 @@ struct TileArgs {
   const TileRec *__restrict__ rec;
   const uint16_t *__restrict__ rec_slot;
   const uint16_t *__restrict__ ext;
+  const int32_t *__restrict__ ext_start;  // [tiles + 1] first extension record of each tile
+  const uint32_t *__restrict__ nreg;      // [tiles] regular records of T | of H1 << 16
   TileGeom<R> geom;
   const uint16_t *__restrict__ flags;
   const R *__restrict__ sfix;
@@ -197,6 +274,9 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
   const int nH1 = A.h1count[tile];
   const int rb = A.rec_start[tile];
+  const int ext0 = A.ext_start[tile];
+  const uint32_t nreg = A.nreg[tile];
+  constexpr int XW = HangExt<D>::WORDS;
   TileRec trec[KT];
   uint32_t tsf[KT];  // LDS slot (low 16 bits) and flags (high 16 bits)
   R tprev[KT];
@@ -261,27 +341,33 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   for (int j = int(threadIdx.x) + KH * BLOCK; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();
   if constexpr (SECOND_ORDER) {
-    auto pass_a = [&](const TileRec &rec, int j) {
+    // rec: the record at tile-local position i, j: its LDS slot
+    auto pass_a = [&](const TileRec &rec, int j, int i) {
       R f000, fnb[2 * D], dd[D];
       if (!(rec.geo & GEO_IRREGULAR)) {
         RegStencil<D, R> st;
         st.load(rec, A.geom);
         laplace<D, R>(st, j, phis, f000, fnb, dd);
       } else {
-        IrrStencil<D, R> st;
-        st.load(rec, A.ext, A.geom);
+        HangStencil<D, R> st;
+        st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
         laplace<D, R>(st, j, phis, f000, fnb, dd);
       }
 #pragma unroll
       for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
     };
 #pragma unroll
-    for (int k = 0; k < KT; ++k)
-      if (int(threadIdx.x) + k * BLOCK < nT) pass_a(trec[k], int(tsf[k] & 0xffffu));
+    for (int k = 0; k < KT; ++k) {
+      const int i = int(threadIdx.x) + k * BLOCK;
+      if (i < nT) pass_a(trec[k], int(tsf[k] & 0xffffu), i);
+    }
 #pragma unroll
-    for (int k = 0; k < K1; ++k)
-      if (nT + int(threadIdx.x) + k * BLOCK < nT + nH1) pass_a(hrec[k], hslot[k]);
-    for (int i = nT + int(threadIdx.x) + K1 * BLOCK; i < nT + nH1; i += BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i]);
+    for (int k = 0; k < K1; ++k) {
+      const int i = nT + int(threadIdx.x) + k * BLOCK;
+      if (i < nT + nH1) pass_a(hrec[k], hslot[k], i);
+    }
+    for (int i = nT + int(threadIdx.x) + K1 * BLOCK; i < nT + nH1; i += BLOCK)
+      pass_a(A.rec[rb + i], A.rec_slot[rb + i], i);
     __syncthreads();
   }
 #pragma unroll
@@ -300,9 +386,9 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
       r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
                                                                    phis, dds, DMAX);
     } else {
-      IrrStencil<D, R> st;
-      st.load(rec, A.ext, A.geom);
-      r = stage_value<D, R, SECOND_ORDER, IrrStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
+      HangStencil<D, R> st;
+      st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
+      r = stage_value<D, R, SECOND_ORDER, HangStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
                                                                    phis, dds, DMAX);
     }
     if constexpr (RK_AVG)
@@ -326,6 +412,9 @@ __device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, d
   const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
   const int nH1 = A.h1count[tile];
   const int rb = A.rec_start[tile];
+  const int ext0 = A.ext_start[tile];
+  const uint32_t nreg = A.nreg[tile];
+  constexpr int XW = HangExt<D>::WORDS;
   for (int j = threadIdx.x; j < nT; j += BLOCK) phis[j] = A.phin[s + j];
   for (int j = threadIdx.x; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();
@@ -339,8 +428,8 @@ __device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, d
         st.load(rec, A.geom);
         laplace<D, double>(st, j, phis, f000, fnb, dd);
       } else {
-        IrrStencil<D, double> st;
-        st.load(rec, A.ext, A.geom);
+        HangStencil<D, double> st;
+        st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
         laplace<D, double>(st, j, phis, f000, fnb, dd);
       }
 #pragma unroll
@@ -359,9 +448,9 @@ __device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, d
       fl = prep_value<D, S, RegStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
                                                                  zdt, SECOND_ORDER, sfix + (s + i), A.N);
     } else {
-      IrrStencil<D, double> st;
-      st.load(rec, A.ext, A.geom);
-      fl = prep_value<D, S, IrrStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
+      HangStencil<D, double> st;
+      st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
+      fl = prep_value<D, S, HangStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
                                                                  zdt, SECOND_ORDER, sfix + (s + i), A.N);
     }
     flags[s + i] = fl;

@@ -12,12 +12,12 @@ from ._lib import CaslError
 from .batch import TreeBatch
 from .forest import DeviceGrid, FlatStencil, Forest, NodeNeighbors, flat_stencil_from_forest
 from .level_set import (DBL_MAX, DeviceStencil, LevelSet, compute_mean_curvature, compute_normals,
-                        my_p4est_level_set_t)
+                        my_p4est_level_set_t, tile_plan)
 
 my_p4est_node_neighbors_t = NodeNeighbors
 
 __all__ = [
     "CaslError", "DBL_MAX", "DeviceGrid", "DeviceStencil", "FlatStencil", "Forest", "LevelSet", "NodeNeighbors", "TreeBatch",
     "compute_mean_curvature", "compute_normals",
-    "flat_stencil_from_forest", "levelset", "my_p4est_level_set_t", "my_p4est_node_neighbors_t",
+    "flat_stencil_from_forest", "levelset", "my_p4est_level_set_t", "my_p4est_node_neighbors_t", "tile_plan",
 ]

@@ -82,6 +82,8 @@ SIGNATURES = {
     "casl_stencil_set_path": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "casl_stencil_tile_info": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int64_p, c_int64_p, c_int64_p,
                                               c_int64_p]),
+    "casl_tile_plan": (ctypes.c_int, [ctypes.POINTER(FlatStencil), c_int64_p, c_int64_p, c_int64_p, c_int64_p,
+                                      c_int64_p]),
     "casl_second_derivatives": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "casl_reinit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),

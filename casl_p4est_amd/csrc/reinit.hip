@@ -607,6 +607,28 @@ int casl_stencil_tile_info(const casl_stencil_t *st, int32_t *tiled, int64_t *n_
   });
 }
 
+int casl_tile_plan(const casl_flat_stencil *tables, int64_t *n_tiles, int64_t *n_records, int64_t *n_irregular,
+                   int64_t *n_irregular_nodes, int64_t *n_two_hanging) {
+  return guarded([&]() {
+    if (!tables) throw std::invalid_argument("null tables");
+    validate_tables(tables);
+    TileHost H;
+    build_tiles(*tables, H);
+    validate_tile_host(H, tables->dim, tables->n_nodes);
+    const int64_t nt = int64_t(H.tile_start.size()) - 1;
+    const int W = tables->dim == 2 ? HangExt<2>::WORDS : HangExt<3>::WORDS;
+    int64_t irr_nodes = 0, two = 0;
+    for (int64_t t = 0; t < nt; ++t) irr_nodes += (H.tile_start[t + 1] - H.tile_start[t]) - int64_t(H.nreg[t] & 0xffffu);
+    if (tables->dim == 3)
+      for (int64_t i = 0; i < H.n_irregular; ++i) two += ((H.ext[size_t(i) * W] >> 4) & 15) != HANG_NONE;
+    if (n_tiles) *n_tiles = nt;
+    if (n_records) *n_records = int64_t(H.rec.size());
+    if (n_irregular) *n_irregular = H.n_irregular;
+    if (n_irregular_nodes) *n_irregular_nodes = irr_nodes;
+    if (n_two_hanging) *n_two_hanging = two;
+  });
+}
+
 int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes) {
   return guarded([&]() {
     if (!st) throw std::invalid_argument("null stencil");

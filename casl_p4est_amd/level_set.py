@@ -31,6 +31,20 @@ def _stream_ptr(torch):
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def tile_plan(tables):
+    """The tiled encoding of host tables, built and checked on the host only
+    (casl_tile_plan; no device needed).  Returns counts of tiles, records
+    (T + H1 over all tiles), records with hanging faces, tile nodes with
+    hanging faces and records with two hanging faces; raises CaslError with
+    the reason when the tables cannot be tiled (their stencil then runs the
+    two-pass path)."""
+    vals = [ctypes.c_int64() for _ in range(5)]
+    s = tables.struct()
+    L.call("casl_tile_plan", ctypes.byref(s), *[ctypes.byref(v) for v in vals])
+    keys = ("n_tiles", "n_records", "n_irregular", "n_irregular_nodes", "n_two_hanging")
+    return {k: v.value for k, v in zip(keys, vals)}
+
+
 class DeviceStencil:
     """Device copy of the flattened stencils (casl_stencil_create)."""
 

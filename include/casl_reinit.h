@@ -109,6 +109,15 @@ int casl_stencil_get_path(const casl_stencil_t *st, int32_t *path);
 int casl_stencil_tile_info(const casl_stencil_t *st, int32_t *tiled,
                            int64_t *n_tiles, int64_t *n_halo,
                            int64_t *n_records, int64_t *n_irregular);
+/* No counterpart in the reference: the tiled encoding of host tables built
+ * and checked on the host alone (no device): counts of tiles, records (T + H1
+ * over all tiles), records with hanging faces, tile nodes with hanging faces
+ * and records with two hanging faces (3D).  Tables that cannot be tiled
+ * return CASL_ERR_INVALID_STENCIL with the reason in casl_last_error; a
+ * stencil made from them runs the two-pass path. */
+int casl_tile_plan(const casl_flat_stencil *tables, int64_t *n_tiles,
+                   int64_t *n_records, int64_t *n_irregular,
+                   int64_t *n_irregular_nodes, int64_t *n_two_hanging);
 
 /* second_derivatives_central: fdd[a*N + n] = d^2 f / dx_a^2 at node n.
  * f and fdd are FP64 device arrays. */

@@ -329,8 +329,9 @@ void validate_tables(const casl_flat_stencil *tab) {
 namespace casl {
 
 // Independent bounds check of every record of a tiled encoding before
-// anything reaches the GPU: tiles within their LDS limits, halo nodes and
-// extension ids in range, every slot used once, every corner slot readable.
+// anything reaches the GPU: tiles within their LDS limits, halo nodes in
+// range, extension ranges matching the irregular records, every slot used
+// once, every corner slot readable, hanging faces valid.
 void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
   const int F = 2 * D, C = 1 << (D - 1);
   const int WORDS = D == 2 ? HangExt<2>::WORDS : HangExt<3>::WORDS, SW = D == 2 ? HangExt<2>::SW : HangExt<3>::SW;

@@ -51,10 +51,10 @@ __global__ void __launch_bounds__(BLOCK) k_advect(Tables<D> t, const double *__r
 // result) / 2.  The loads a node needs (its record, slot and vn, and the
 // value being averaged) are issued before the first barrier.
 template <int D, bool RK_AVG>
-__global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu(CASL_TILE_WAVES_F64)))
+__global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_per_eu(CASL_TILE_WAVES_F64)))
     k_tile_advect(TileArgs<double> A, const double *__restrict__ vn, double dt) {
   constexpr int DMAX = TileLimits<D>::DMAX;
-  constexpr int KT = TileLimits<D>::TMAX / TILE_BLOCK;
+  constexpr int KT = TileLimits<D>::TMAX / TILE_BLOCK_F64;
   __shared__ double phis[TileLimits<D>::LMAX];
   __shared__ double dds[D * DMAX];
   const int tile = xcd_tile(int(blockIdx.x), int(gridDim.x));
@@ -70,17 +70,17 @@ __global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu
   double tvn[KT], tprev[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
-    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    const int i = int(threadIdx.x) + k * TILE_BLOCK_F64;
     if (i < nT) {
       trec[k] = A.rec[rb + i];
       tslot[k] = A.rec_slot[rb + i];
     }
   }
-  for (int j = threadIdx.x; j < nT; j += TILE_BLOCK) phis[j] = A.phin[s + j];
-  for (int j = threadIdx.x; j < nH; j += TILE_BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
+  for (int j = threadIdx.x; j < nT; j += TILE_BLOCK_F64) phis[j] = A.phin[s + j];
+  for (int j = threadIdx.x; j < nH; j += TILE_BLOCK_F64) phis[nT + j] = A.phin[A.halo[hb + j]];
 #pragma unroll
   for (int k = 0; k < KT; ++k)
-    if (int(threadIdx.x) + k * TILE_BLOCK < nT) {
+    if (int(threadIdx.x) + k * TILE_BLOCK_F64 < nT) {
       tvn[k] = vn[s + tslot[k]];
       if constexpr (RK_AVG) tprev[k] = A.out[s + tslot[k]];
     }
@@ -102,14 +102,14 @@ __global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu
   };
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
-    const int i = int(threadIdx.x) + k * TILE_BLOCK;
+    const int i = int(threadIdx.x) + k * TILE_BLOCK_F64;
     if (i < nT) pass_a(trec[k], tslot[k], i);
   }
-  for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK) pass_a(A.rec[rb + i], A.rec_slot[rb + i], i);
+  for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK_F64) pass_a(A.rec[rb + i], A.rec_slot[rb + i], i);
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
-    if (int(threadIdx.x) + k * TILE_BLOCK >= nT) continue;
+    if (int(threadIdx.x) + k * TILE_BLOCK_F64 >= nT) continue;
     const TileRec &rec = trec[k];
     const int j = tslot[k];
     const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
@@ -120,7 +120,7 @@ __global__ void __launch_bounds__(TILE_BLOCK) __attribute__((amdgpu_waves_per_eu
       r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
     } else {
       HangStencil<D, double> st;
-      st.load(rec, hang_record(A.ext, ext0, nreg, nT, int(threadIdx.x) + k * TILE_BLOCK, XW), A.geom);
+      st.load(rec, hang_record(A.ext, ext0, nreg, nT, int(threadIdx.x) + k * TILE_BLOCK_F64, XW), A.geom);
       r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
     }
     if constexpr (RK_AVG)
@@ -161,7 +161,7 @@ void run_advect(casl_stencil *st, double *phi, const double *vn, const double *v
   if (st->tiled && st->path == CASL_PATH_TILED) {
     st->phi0.ensure(N);
     TileArgs<double> ta = tile_args<double>(st);
-    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK);
+    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK_F64);
     ta.phin = phi;
     ta.out = st->phi0.p;
     prof_begin("k_tile_advect", s);

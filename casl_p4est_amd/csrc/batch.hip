@@ -316,7 +316,7 @@ void launch_batch(casl_batch *B, double *phi, int n_iter, hipStream_t s, bool al
     // flags and s_fix of every tree in tile-record order (FP64 tiled prep)
     TileArgs<double> tp = tile_args<double>(&st);
     tp.phin = st.phi0.p;
-    hipLaunchKernelGGL((k_tile_prep<D, R, true>), dim3(unsigned(st.n_tiles)), dim3(TILE_BLOCK), 0, s, tp,
+    hipLaunchKernelGGL((k_tile_prep<D, R, true>), dim3(unsigned(st.n_tiles)), dim3(TILE_BLOCK_F64), 0, s, tp,
                        (const uint8_t *)nullptr, -DBL_MAX, DBL_MAX, st.zdt, st.flags.p, A.sfix);
     HIP_CHECK(hipGetLastError());
     TileKernelFn<D, R> fn = pick_tile_kernel<D, R>(B->sched.block);

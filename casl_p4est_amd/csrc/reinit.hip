@@ -219,7 +219,7 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
     // flags and s_fix in tile-record order, the order the tiled stages read them
     TileArgs<double> tp = tile_args<double>(st);
     tp.phin = st->phi0.p;
-    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK);
+    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK_F64);
     prof_begin("k_tile_prep", s);
     if (so)
       hipLaunchKernelGGL((k_tile_prep<D, R, true>), g, b, 0, s, tp, mask, lim_low, lim_high, st->zdt, st->flags.p,
@@ -273,7 +273,7 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   auto tile_stage = [&](const R *in, R *out, bool avg) {
     ta.phin = in;
     ta.out = out;
-    const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK);
+    const dim3 g(unsigned(st->n_tiles)), b(stage_block<R>());
     prof_begin("k_tile_stage", s);
     if (so) {
       if (avg)

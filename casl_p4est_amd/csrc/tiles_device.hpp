@@ -239,10 +239,23 @@ struct TileArgs {
   int64_t N;
 };
 
+// Threads per block of the tile kernels.  The FP64 kernels hold 160 KB of
+// LDS arrays, so one block runs per compute unit: a 1024-thread block (2 tile
+// nodes per thread, at most 128 VGPRs) keeps 4 waves per SIMD where a
+// 512-thread one kept 2.  The FP32 stage kernel's 80 KB let two 512-thread
+// blocks share a compute unit.
 #ifndef CASL_TILE_BLOCK
 #define CASL_TILE_BLOCK 512
 #endif
+#ifndef CASL_TILE_BLOCK_F64
+#define CASL_TILE_BLOCK_F64 1024
+#endif
 constexpr int TILE_BLOCK = CASL_TILE_BLOCK;
+constexpr int TILE_BLOCK_F64 = CASL_TILE_BLOCK_F64;
+template <class R>
+constexpr int stage_block() {
+  return sizeof(R) == 8 ? TILE_BLOCK_F64 : TILE_BLOCK;
+}
 
 // One pseudo-time stage of a tile: phi of T u H1 u H2 into LDS, pass A for
 // T u H1 in LDS, pass B for T.  out = result, or (RK_AVG) out = (out + result)/2.
@@ -258,7 +271,7 @@ constexpr int TILE_BLOCK = CASL_TILE_BLOCK;
 #define CASL_TILE_WAVES_F32 4
 #endif
 #ifndef CASL_TILE_WAVES_F64
-#define CASL_TILE_WAVES_F64 3
+#define CASL_TILE_WAVES_F64 4
 #endif
 // The stage body for one tile, run by a block of BLOCK threads with LDS
 // arrays phis[LMAX] and dds[D * DMAX].  It reads and writes global memory and
@@ -305,7 +318,8 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   // the phi gather, unrolled so that a thread's loads are in flight together:
   // the halo ids of the first KH rows, then every phi value, then the LDS
   // stores (the rare tail beyond KH rows of halo takes the plain loop)
-  constexpr int KH = (TileLimits<D>::LMAX - TileLimits<D>::TMAX) / BLOCK / 2;
+  constexpr int KH0 = (TileLimits<D>::LMAX - TileLimits<D>::TMAX) / BLOCK / 2;
+  constexpr int KH = KH0 > 0 ? KH0 : 1;
   int hid[KH];
 #pragma unroll
   for (int k = 0; k < KH; ++k) {
@@ -458,22 +472,23 @@ __device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, d
 }
 
 template <int D, class S, bool SECOND_ORDER>
-__global__ void __launch_bounds__(TILE_BLOCK) k_tile_prep(TileArgs<double> A, const uint8_t *__restrict__ mask,
+__global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_per_eu(CASL_TILE_WAVES_F64)))
+    k_tile_prep(TileArgs<double> A, const uint8_t *__restrict__ mask,
                                                           double lim_low, double lim_high, double zdt,
                                                           uint16_t *__restrict__ flags, S *__restrict__ sfix) {
   __shared__ double phis[TileLimits<D>::LMAX];
   __shared__ double dds[SECOND_ORDER ? D * TileLimits<D>::DMAX : 1];
-  tile_prep<D, S, SECOND_ORDER, TILE_BLOCK>(A, xcd_tile(int(blockIdx.x), int(gridDim.x)), phis, dds, mask, lim_low,
+  tile_prep<D, S, SECOND_ORDER, TILE_BLOCK_F64>(A, xcd_tile(int(blockIdx.x), int(gridDim.x)), phis, dds, mask, lim_low,
                                             lim_high, zdt, flags, sfix);
 }
 
 template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
-__global__ void __launch_bounds__(TILE_BLOCK)
+__global__ void __launch_bounds__(stage_block<R>())
     __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
     k_tile_stage(TileArgs<R> A) {
   __shared__ R phis[TileLimits<D>::LMAX];
   __shared__ R dds[SECOND_ORDER ? D * TileLimits<D>::DMAX : 1];
-  tile_stage<D, R, SECOND_ORDER, RK_AVG, TILE_BLOCK>(A, xcd_tile(int(blockIdx.x), int(gridDim.x)), phis, dds);
+  tile_stage<D, R, SECOND_ORDER, RK_AVG, stage_block<R>()>(A, xcd_tile(int(blockIdx.x), int(gridDim.x)), phis, dds);
 }
 
 }  // namespace dev

@@ -142,11 +142,16 @@ __global__ void k_batch_copy(const double *__restrict__ phi, double *__restrict_
   cur[n] = R(v);
 }
 
+// Register caps of k_batch_tiles (waves per SIMD): its 256- and 512-thread
+// blocks run four or two tile nodes per thread, which need up to 128 VGPRs
+// in FP32 and 168 in FP64.
+constexpr int BATCH_WAVES_F32 = 4, BATCH_WAVES_F64 = 3;
+
 // Persistent kernel over trees whose pseudo-time stages run tile by tile in
 // LDS; every tree's result is its single-tree tiled result.
 template <int D, class R, int BLOCK>
 __global__ void __launch_bounds__(BLOCK)
-    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? CASL_TILE_WAVES_F32 : CASL_TILE_WAVES_F64)))
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? BATCH_WAVES_F32 : BATCH_WAVES_F64)))
     k_batch_tiles(BatchTileArgs<R> A) {
   __shared__ R phis[TileLimits<D>::LMAX];
   __shared__ R dds[D * TileLimits<D>::DMAX];

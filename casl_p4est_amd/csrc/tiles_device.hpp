@@ -239,13 +239,14 @@ struct TileArgs {
   int64_t N;
 };
 
-// Threads per block of the tile kernels.  The FP64 kernels hold 160 KB of
-// LDS arrays, so one block runs per compute unit: a 1024-thread block (2 tile
-// nodes per thread, at most 128 VGPRs) keeps 4 waves per SIMD where a
-// 512-thread one kept 2.  The FP32 stage kernel's 80 KB let two 512-thread
-// blocks share a compute unit.
+// Threads per block of the tile kernels: 1024, two tile nodes per thread.
+// The FP64 kernels hold 160 KB of LDS arrays, so one block runs per compute
+// unit, at most 128 VGPRs for 4 waves per SIMD (a 512-thread block kept 2).
+// The FP32 stage kernel holds 80 KB, so two blocks share a compute unit, at
+// most 64 VGPRs for 8 waves per SIMD (two 512-thread blocks of four nodes
+// per thread kept 4).
 #ifndef CASL_TILE_BLOCK
-#define CASL_TILE_BLOCK 512
+#define CASL_TILE_BLOCK 1024
 #endif
 #ifndef CASL_TILE_BLOCK_F64
 #define CASL_TILE_BLOCK_F64 1024
@@ -268,7 +269,7 @@ constexpr int stage_block() {
 // registers and LDS (crossing arms aside, whose s_fix is read in pass B).
 // Occupancy: the register budget is capped to keep TILE_WAVES_* waves per SIMD.
 #ifndef CASL_TILE_WAVES_F32
-#define CASL_TILE_WAVES_F32 4
+#define CASL_TILE_WAVES_F32 8
 #endif
 #ifndef CASL_TILE_WAVES_F64
 #define CASL_TILE_WAVES_F64 4

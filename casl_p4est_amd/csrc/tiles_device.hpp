@@ -261,12 +261,14 @@ constexpr int stage_block() {
 // One pseudo-time stage of a tile: phi of T u H1 u H2 into LDS, pass A for
 // T u H1 in LDS, pass B for T.  out = result, or (RK_AVG) out = (out + result)/2.
 //
-// Latency: every global load a node's work needs is issued before the first
+// Latency: the global loads of a node's stencil are issued before the first
 // barrier -- each thread's tile records (its own nodes, used in both passes),
-// their LDS slots, their flags (stored in record order by the prep pass, so
-// the address does not wait for the slot) and, for the second RK2 stage, the
-// phi_n value being averaged -- so after the phi gather the passes run on
-// registers and LDS (crossing arms aside, whose s_fix is read in pass B).
+// their LDS slots and their flags (stored in record order by the prep pass,
+// so the address does not wait for the slot) -- so after the phi gather the
+// passes run on registers and LDS (crossing arms aside, whose s_fix is read
+// in pass B).  The second RK2 stage reads the phi_n value it averages with
+// at the store: held from the start it cost two spilled registers under the
+// FP32 cap (3 % slower).
 // Occupancy: the register budget is capped to keep TILE_WAVES_* waves per SIMD.
 #ifndef CASL_TILE_WAVES_F32
 #define CASL_TILE_WAVES_F32 8
@@ -293,7 +295,6 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   constexpr int XW = HangExt<D>::WORDS;
   TileRec trec[KT];
   uint32_t tsf[KT];  // LDS slot (low 16 bits) and flags (high 16 bits)
-  R tprev[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * BLOCK;
@@ -337,11 +338,6 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   for (int k = 0; k < KH; ++k) {
     const int j = int(threadIdx.x) + k * BLOCK;
     if (j < nH) hv[k] = A.phin[hid[k]];
-  }
-  if constexpr (RK_AVG) {
-#pragma unroll
-    for (int k = 0; k < KT; ++k)
-      if (int(threadIdx.x) + k * BLOCK < nT) tprev[k] = A.out[s + int(tsf[k] & 0xffffu)];
   }
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
@@ -407,7 +403,7 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
                                                                    phis, dds, DMAX);
     }
     if constexpr (RK_AVG)
-      A.out[n] = R(0.5) * (tprev[k] + r);
+      A.out[n] = R(0.5) * (A.out[n] + r);  // phi_n, read here rather than held in registers
     else
       A.out[n] = r;
   }

@@ -67,7 +67,6 @@ __global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_pe
   constexpr int XW = HangExt<D>::WORDS;
   TileRec trec[KT];
   int tslot[KT];
-  double tvn[KT], tprev[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * TILE_BLOCK_F64;
@@ -78,12 +77,6 @@ __global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_pe
   }
   for (int j = threadIdx.x; j < nT; j += TILE_BLOCK_F64) phis[j] = A.phin[s + j];
   for (int j = threadIdx.x; j < nH; j += TILE_BLOCK_F64) phis[nT + j] = A.phin[A.halo[hb + j]];
-#pragma unroll
-  for (int k = 0; k < KT; ++k)
-    if (int(threadIdx.x) + k * TILE_BLOCK_F64 < nT) {
-      tvn[k] = vn[s + tslot[k]];
-      if constexpr (RK_AVG) tprev[k] = A.out[s + tslot[k]];
-    }
   __syncthreads();
   // rec: the record at tile-local position i, j: its LDS slot
   auto pass_a = [&](const TileRec &rec, int j, int i) {
@@ -117,14 +110,16 @@ __global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_pe
     if (!(rec.geo & GEO_IRREGULAR)) {
       RegStencil<D, double> st;
       st.load(rec, A.geom);
-      r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
+      r = advect_value<D>(st, j, w, vn[s + j], dt, phis, dds, DMAX);
     } else {
       HangStencil<D, double> st;
       st.load(rec, hang_record(A.ext, ext0, nreg, nT, int(threadIdx.x) + k * TILE_BLOCK_F64, XW), A.geom);
-      r = advect_value<D>(st, j, w, tvn[k], dt, phis, dds, DMAX);
+      r = advect_value<D>(st, j, w, vn[s + j], dt, phis, dds, DMAX);
     }
+    // vn and the phi_n being averaged are read here rather than held in
+    // registers through the passes (the 128-VGPR cap would spill them)
     if constexpr (RK_AVG)
-      A.out[s + j] = 0.5 * (tprev[k] + r);
+      A.out[s + j] = 0.5 * (A.out[s + j] + r);
     else
       A.out[s + j] = r;
   }

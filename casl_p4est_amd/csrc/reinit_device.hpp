@@ -286,8 +286,8 @@ __device__ __forceinline__ R stage_value(const TB &t, int64_t j, uint16_t fl, ui
   // arm lengths are positive: fmin is MIN's value, in one instruction
   R dt = fmin(fmin(s[0], s[1]), fmin(s[2], s[3]));
   if constexpr (D == 3) dt = fmin(dt, fmin(s[4], s[5]));
-  // FP64 keeps the reference's division; in FP32 (the mixed path, judged
-  // against FP64 at 1e-3 h) a multiplication by 1/D differs by an ulp at most
+  // FP64 keeps the reference's division; in FP32 (the mixed path, held
+  // to 1e-3 h of FP64) a multiplication by 1/D differs by an ulp at most
   if constexpr (sizeof(R) == 4)
     dt *= R(1.0 / D);
   else

@@ -80,7 +80,7 @@ __device__ __forceinline__ double interface_location_2nd(double a, double b, dou
 }
 
 // x / y.  FP64 divides (the path that reproduces the reference bit for bit);
-// FP32 -- the mixed path, judged against FP64 at 1e-3 h -- multiplies by the
+// FP32 -- the mixed path, held to 1e-3 h of FP64 -- multiplies by the
 // hardware reciprocal (about 1 ulp), two instructions instead of a range-safe
 // division sequence.
 template <class R>

@@ -266,9 +266,7 @@ constexpr int stage_block() {
 // their LDS slots and their flags (stored in record order by the prep pass,
 // so the address does not wait for the slot) -- so after the phi gather the
 // passes run on registers and LDS (crossing arms aside, whose s_fix is read
-// in pass B).  The second RK2 stage reads the phi_n value it averages with
-// at the store: held from the start it cost two spilled registers under the
-// FP32 cap (3 % slower).
+// in pass B).  The second RK2 stage's phi_n: see HOLD_PREV.
 // Occupancy: the register budget is capped to keep TILE_WAVES_* waves per SIMD.
 #ifndef CASL_TILE_WAVES_F32
 #define CASL_TILE_WAVES_F32 8
@@ -295,6 +293,12 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   constexpr int XW = HangExt<D>::WORDS;
   TileRec trec[KT];
   uint32_t tsf[KT];  // LDS slot (low 16 bits) and flags (high 16 bits)
+  // the phi_n averaged with in the second RK2 stage: loaded before the first
+  // barrier where registers allow (blocks below 1024 threads run under a
+  // 128-VGPR cap), else read at the store (under the 64-VGPR cap of the
+  // 1024-thread FP32 kernel holding it costs spills)
+  constexpr bool HOLD_PREV = RK_AVG && BLOCK < 1024;
+  R tprev[HOLD_PREV ? KT : 1];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * BLOCK;
@@ -333,6 +337,11 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   for (int k = 0; k < KT; ++k) {
     const int j = int(threadIdx.x) + k * BLOCK;
     if (j < nT) tv[k] = A.phin[s + j];
+  }
+  if constexpr (HOLD_PREV) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k)
+      if (int(threadIdx.x) + k * BLOCK < nT) tprev[k] = A.out[s + int(tsf[k] & 0xffffu)];
   }
 #pragma unroll
   for (int k = 0; k < KH; ++k) {
@@ -402,8 +411,10 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
       r = stage_value<D, R, SECOND_ORDER, HangStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
                                                                    phis, dds, DMAX);
     }
-    if constexpr (RK_AVG)
-      A.out[n] = R(0.5) * (A.out[n] + r);  // phi_n, read here rather than held in registers
+    if constexpr (HOLD_PREV)
+      A.out[n] = R(0.5) * (tprev[k] + r);
+    else if constexpr (RK_AVG)
+      A.out[n] = R(0.5) * (A.out[n] + r);
     else
       A.out[n] = r;
   }

@@ -26,7 +26,8 @@ def hanging_faces(tab):
     (2, ls.Sphere([0.5, 0.5], 0.3), 1, 10),
     (3, ls.random_spheres(12, seed=4, rmin=0.06, rmax=0.12), 2, 7),
     (3, ls.Sphere([0.45, 0.5, 0.55], 0.3), 1, 7),
-], ids=["star2d", "circle2d_jump", "spheres3d", "sphere3d_jump"])
+    (3, ls.Gyroid(), 4, 6),  # hanging faces at wall nodes too
+], ids=["star2d", "circle2d_jump", "spheres3d", "sphere3d_jump", "gyroid3d_walls"])
 def test_plan_counts_match_tables(dim, shape, lmin, lmax):
     f = C.Forest(dim).refine_levelset(shape, lmin, lmax, lip=2.4).build_nodes()
     tab = C.flat_stencil_from_forest(f)

@@ -38,7 +38,7 @@ def problem(dim, shape, lmin, lmax, lip=2.4):
 PROBLEMS = {
     "star2d": (2, ls.Star(), 3, 8),
     "spheres3d": (3, ls.random_spheres(6, seed=5, rmin=0.06, rmax=0.14), 2, 6),
-    # 222K nodes, 43 % of the records with hanging faces (a third of those
+    # 222K nodes, 29 % of the records with hanging faces (a quarter of those
     # with two), hanging faces at wall nodes
     "gyroid3d": (3, ls.Gyroid(), 4, 6),
 }

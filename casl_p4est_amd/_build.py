@@ -36,7 +36,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", INCLUDE, "-I", CSRC]
 
 HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip", "curv.hip", "derivatives.hip", "advect.hip", "gpu_forest.hip",
-               "curv_grid.hip"]
+               "curv_grid.hip", "tiles_gpu.hip"]
 HOST_SOURCES = ["forest.cpp", "tiles.cpp"]
 
 
@@ -64,6 +64,17 @@ def build_library(verbose=True):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-pthread"], verbose)
     os.replace(tmp, LIB)
     return LIB
+
+
+def build_c_caller(verbose=True):
+    """The test program that calls the library from C++ through
+    include/casl_reinit.h alone (tests/c_abi), compiled with the host g++."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "c_abi", "reinit_caller.cpp")
+    out = os.path.join(root, "tests", "c_abi", "reinit_caller")
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", INCLUDE, src, "-o", out, "-L", HERE, "-lcasl_reinit",
+          "-Wl,-rpath,$ORIGIN/../../casl_p4est_amd"], verbose)
+    return out
 
 
 def build_oracle(verbose=True):

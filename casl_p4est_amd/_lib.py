@@ -84,9 +84,13 @@ SIGNATURES = {
                                               c_int64_p]),
     "casl_tile_plan": (ctypes.c_int, [ctypes.POINTER(FlatStencil), c_int64_p, c_int64_p, c_int64_p, c_int64_p,
                                       c_int64_p]),
+    "casl_stencil_tile_check": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FlatStencil), c_int32_p]),
     "casl_second_derivatives": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "casl_reinit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),
+    "casl_reinit_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_void_p]),
     "casl_reinit_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),
     "casl_reinit_masked": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,

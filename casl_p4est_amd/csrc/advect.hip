@@ -58,63 +58,53 @@ __global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_pe
   __shared__ double phis[TileLimits<D>::LMAX];
   __shared__ double dds[D * DMAX];
   const int tile = xcd_tile(int(blockIdx.x), int(gridDim.x));
-  const int s = A.tile_start[tile], nT = A.tile_start[tile + 1] - s;
-  const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
-  const int nH1 = A.h1count[tile];
-  const int rb = A.rec_start[tile];
-  const int ext0 = A.ext_start[tile];
-  const uint32_t nreg = A.nreg[tile];
-  constexpr int XW = HangExt<D>::WORDS;
-  TileRec trec[KT];
+  TileCtx<D> c;
+  c.load(A, tile);
+  const int s = c.s, nT = c.nT, hb = c.hb, nH = c.nH, nH1 = c.nH1;
+  const bool ufast = A.geom.pow2 != 0;
+  RecRegs trec[KT];
   int tslot[KT];
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * TILE_BLOCK_F64;
-    if (i < nT) {
-      trec[k] = A.rec[rb + i];
-      tslot[k] = A.rec_slot[rb + i];
-    }
+    if (i < nT) tslot[k] = fetch_record<D>(A, c, i, trec[k]);
   }
   for (int j = threadIdx.x; j < nT; j += TILE_BLOCK_F64) phis[j] = A.phin[s + j];
   for (int j = threadIdx.x; j < nH; j += TILE_BLOCK_F64) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();
-  // rec: the record at tile-local position i, j: its LDS slot
-  auto pass_a = [&](const TileRec &rec, int j, int i) {
-    double f000, fnb[2 * D], dd[D];
-    if (!(rec.geo & GEO_IRREGULAR)) {
-      RegStencil<D, double> st;
-      st.load(rec, A.geom);
-      laplace<D, double>(st, j, phis, f000, fnb, dd);
-    } else {
-      HangStencil<D, double> st;
-      st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
-      laplace<D, double>(st, j, phis, f000, fnb, dd);
-    }
-#pragma unroll
-    for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
-  };
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * TILE_BLOCK_F64;
-    if (i < nT) pass_a(trec[k], tslot[k], i);
+    if (i < nT) tile_pass_a<D, double>(A, c, ufast, trec[k], tslot[k], i, phis, dds);
   }
-  for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK_F64) pass_a(A.rec[rb + i], A.rec_slot[rb + i], i);
+  for (int i = nT + int(threadIdx.x); i < nT + nH1; i += TILE_BLOCK_F64) {
+    RecRegs r;
+    const int js = fetch_record<D>(A, c, i, r);
+    tile_pass_a<D, double>(A, c, ufast, r, js, i, phis, dds);
+  }
   __syncthreads();
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
-    if (int(threadIdx.x) + k * TILE_BLOCK_F64 >= nT) continue;
-    const TileRec &rec = trec[k];
-    const int j = tslot[k];
-    const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
+    const int i = int(threadIdx.x) + k * TILE_BLOCK_F64;
+    if (i >= nT) continue;
+    int j = tslot[k];
     double r;
-    if (!(rec.geo & GEO_IRREGULAR)) {
-      RegStencil<D, double> st;
-      st.load(rec, A.geom);
-      r = advect_value<D>(st, j, w, vn[s + j], dt, phis, dds, DMAX);
+    if (ufast && c.uniform(i)) {
+      UniStencil<D, double> st;
+      j = st.load(trec[k].w, A.geom);
+      r = advect_value<D>(st, j, 0, vn[s + j], dt, phis, dds, DMAX);
     } else {
-      HangStencil<D, double> st;
-      st.load(rec, hang_record(A.ext, ext0, nreg, nT, int(threadIdx.x) + k * TILE_BLOCK_F64, XW), A.geom);
-      r = advect_value<D>(st, j, w, vn[s + j], dt, phis, dds, DMAX);
+      const TileRec rec = record_at<D>(c, i, trec[k], j);
+      const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
+      if (!(rec.geo & GEO_IRREGULAR)) {
+        RegStencil<D, double> st;
+        st.load(rec, A.geom);
+        r = advect_value<D>(st, j, w, vn[s + j], dt, phis, dds, DMAX);
+      } else {
+        HangStencil<D, double> st;
+        st.load(rec, c.ext_of(A.ext, i), A.geom);
+        r = advect_value<D>(st, j, w, vn[s + j], dt, phis, dds, DMAX);
+      }
     }
     // vn and the phi_n being averaged are read here rather than held in
     // registers through the passes (the 128-VGPR cap would spill them)

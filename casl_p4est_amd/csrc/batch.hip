@@ -389,6 +389,8 @@ void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std:
   all.halo_start.push_back(0);
   all.rec_start.push_back(0);
   all.ext_start.push_back(0);
+  all.urec_start.push_back(0);
+  const int UW = D == 2 ? UniRec<2>::WORDS : UniRec<3>::WORDS;
   std::vector<int32_t> tree_tile(static_cast<size_t>(nt) + 1, 0);
   const int C = 1 << (D - 1);
   for (int32_t k = 0; k < nt; ++k) {
@@ -397,6 +399,7 @@ void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std:
     const int32_t halo_off = int32_t(all.halo_idx.size());
     const int32_t rec_off = int32_t(all.rec.size());
     const int32_t ext_off = int32_t(all.ext.size() / size_t(WORDS));
+    const int32_t urec_off = int32_t(all.urec.size() / size_t(UW));
     const int64_t ntile = int64_t(H.tile_start.size()) - 1;
     tree_tile[size_t(k)] = int32_t(all.tile_start.size()) - 1;
     for (int64_t t = 0; t < ntile; ++t) {
@@ -406,7 +409,12 @@ void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std:
       all.h1count.push_back(H.h1count[size_t(t)]);
       all.ext_start.push_back(H.ext_start[size_t(t) + 1] + ext_off);
       all.nreg.push_back(H.nreg[size_t(t)]);
+      all.urec_start.push_back(H.urec_start[size_t(t) + 1] + urec_off);
+      all.nuni.push_back(H.nuni[size_t(t)]);
     }
+    // uniform records hold tile-local indices and levels of the shared
+    // coarsest cell only: they need no offset
+    all.urec.insert(all.urec.end(), H.urec.begin(), H.urec.end());
     for (int32_t g : H.halo_idx) all.halo_idx.push_back(g + node_off);
     all.rec.insert(all.rec.end(), H.rec.begin(), H.rec.end());
     all.rec_slot.insert(all.rec_slot.end(), H.rec_slot.begin(), H.rec_slot.end());

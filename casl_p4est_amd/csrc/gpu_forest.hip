@@ -711,26 +711,20 @@ int casl_device_grid_stencil(const casl_device_grid_t *g, casl_stencil_t **out) 
   return guarded([&]() {
     if (!g || !out) throw std::invalid_argument("null argument");
     *out = nullptr;
-    // The tiled encoding is built on the host, so the tables travel there once.
-    const int D = g->dim, F = 2 * D, C = 1 << (D - 1), T = 2 * (D - 1);
-    const size_t N = size_t(g->N);
-    std::vector<int32_t> nbr(size_t(F) * C * N);
-    std::vector<double> d(size_t(F) * N), dtr(size_t(F) * T * N);
-    std::vector<uint8_t> wall(N);
-    HIP_CHECK(hipMemcpy(nbr.data(), g->nbr.p, nbr.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(d.data(), g->dface.p, d.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(dtr.data(), g->dtr.p, dtr.size() * sizeof(double), hipMemcpyDeviceToHost));
-    HIP_CHECK(hipMemcpy(wall.data(), g->wall.p, N, hipMemcpyDeviceToHost));
+    // the grid-wide tables and the tiled encoding, both from the grid's device
+    // tables: nothing travels to the host but a few counts
     casl_flat_stencil tab{};
-    tab.dim = D;
+    tab.dim = g->dim;
     tab.n_nodes = g->N;
-    tab.nbr = nbr.data();
-    tab.d = d.data();
-    tab.d_tr = dtr.data();
-    tab.wall = wall.data();
+    tab.nbr = g->nbr.p;
+    tab.d = g->dface.p;
+    tab.d_tr = g->dtr.p;
+    tab.wall = g->wall.p;
     tab.zero_distance_threshold = g->zdt;
     auto st = std::make_unique<casl_stencil>();
-    upload_tables(&tab, st.get(), true);
+    upload_device_tables(tab, st.get(), nullptr);
+    build_tiles_device(tab, st.get(), nullptr);
+    HIP_CHECK(hipStreamSynchronize(nullptr));
     *out = st.release();
   });
 }

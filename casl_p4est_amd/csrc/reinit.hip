@@ -23,8 +23,10 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <vector>
 
+#include "common.hpp"
 #include "reinit_device.hpp"
 #include "tiles_device.hpp"
 
@@ -74,21 +76,50 @@ __global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, int64
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) b[i] = float(a[i]);
 }
-__global__ void k_f2d(const float *__restrict__ a, double *__restrict__ b, int64_t n) {
+// The mixed path's way back to FP64.  Nodes the stages never update keep
+// their FP64 input bit for bit, as in the reference: outside the phi_0 limits
+// phi_np1 = phi_0 (src/my_p4est_level_set.cpp:277-278), masked nodes keep
+// their current value, which is the input (phi_0) at every stage.
+__device__ __forceinline__ double mixed_result(float v, uint16_t fl, double p0) {
+  const int mode = fl & 3;
+  return (mode == MODE_COPY0 || mode == MODE_MASKED) ? p0 : double(v);
+}
+// flags in node order (grid-wide path)
+__global__ void k_f2d_keep(const float *__restrict__ a, const uint16_t *__restrict__ flags,
+                           const double *__restrict__ phi0, double *__restrict__ b, int64_t n) {
   const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < n) b[i] = double(a[i]);
+  if (i < n) b[i] = mixed_result(a[i], flags[i], phi0[i]);
+}
+// flags in tile-record order (tiled path): one block per tile
+template <int D>
+__global__ void __launch_bounds__(BLOCK)
+    k_tile_f2d_keep(TileArgs<float> T, const float *__restrict__ a, const uint16_t *__restrict__ flags,
+                    const double *__restrict__ phi0, double *__restrict__ b) {
+  TileCtx<D> c;
+  c.load(T, int(blockIdx.x));
+  for (int i = int(threadIdx.x); i < c.nT; i += BLOCK) {
+    RecRegs r;
+    const int js = fetch_record<D>(T, c, i, r);
+    const int64_t n = int64_t(c.s) + slot_at<D>(c, i, r, js);
+    b[n] = mixed_result(a[n], flags[c.s + i], phi0[n]);
+  }
 }
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 // max |a - b| over nodes with |ref| < band; non-negative doubles order like
-// their bit patterns, so an unsigned 64-bit atomicMax reduces them.
+// their bit patterns, so an unsigned 64-bit atomicMax reduces them.  A NaN
+// difference counts as +inf (fmax would drop it), so a NaN in the band cannot
+// hide behind a small error.
 __global__ void __launch_bounds__(BLOCK) k_band_linf(const double *__restrict__ a, const double *__restrict__ b,
                                                      const double *__restrict__ ref, double band, int64_t n,
                                                      unsigned long long *__restrict__ result) {
   __shared__ double red[BLOCK];
   double m = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
-    if (fabs(ref[i]) < band) m = fmax(m, fabs(a[i] - b[i]));
+    if (fabs(ref[i]) < band) {
+      const double d = fabs(a[i] - b[i]);
+      m = fmax(m, d == d ? d : double(INFINITY));
+    }
   red[threadIdx.x] = m;
   __syncthreads();
   for (int s = BLOCK / 2; s > 0; s >>= 1) {
@@ -136,6 +167,9 @@ dev::TileArgs<R> tile_args(const casl_stencil *st) {
   ta.ext = st->t_ext.p;
   ta.ext_start = st->t_ext_start.p;
   ta.nreg = st->t_nreg.p;
+  ta.urec = st->t_urec.p;
+  ta.urec_start = st->t_urec_start.p;
+  ta.nuni = st->t_nuni.p;
   ta.geom.pow2 = 1;
   for (int a = 0; a < 3; ++a) {
     ta.geom.base[a] = R(st->geom_base[a]);
@@ -160,78 +194,115 @@ namespace {
 
 // reinitialize_within_range_of_phi_0 driver, templated on dimension and the
 // stage arithmetic type R (double: FP64, float: mixed).
-// Every scratch buffer a reinitialization call of this kind uses, allocated
-// up front (a recorded graph must not allocate).
+
+// The device buffers of one call: the FP64 copy of phi_0, the per-node flags
+// and subcell distances (tile-record order on the tiled path), the stage
+// buffers (mixed: the FP32 phi and its partner; FP64: phi itself and tmp)
+// and, on the two-pass path, the pass-A outputs.
+template <class R>
+struct ReinitBufs {
+  double *phi0 = nullptr;
+  uint16_t *flags = nullptr;
+  double *dd0 = nullptr;  // two-pass, second order: pass A of phi_0
+  R *sfix = nullptr;      // [F][N]
+  R *work = nullptr;      // mixed only: FP32 phi
+  R *tmp = nullptr;
+  R *ddn = nullptr;       // two-pass, second order: pass A of the stage input
+};
+
+// Sizes (elements) of the buffers a call of this kind uses; zero when unused.
+struct ReinitSizes {
+  size_t phi0, flags, dd0, sfix, work, tmp, ddn;
+};
 template <int D, class R>
-void reinit_scratch(casl_stencil *st, bool so, bool tiled) {
-  const size_t N = size_t(st->N);
+ReinitSizes reinit_sizes(size_t N, bool so, bool tiled) {
   constexpr int F = 2 * D;
-  st->phi0.ensure(N);
-  st->flags.ensure(N);
-  if (so && !tiled) st->dd0.ensure(size_t(D) * N);
+  ReinitSizes z{};
+  z.phi0 = N;
+  z.flags = N;
+  z.dd0 = (so && !tiled) ? size_t(D) * N : 0;
+  z.sfix = size_t(F) * N;
+  z.work = sizeof(R) == 4 ? N : 0;
+  z.tmp = N;
+  z.ddn = (so && !tiled) ? size_t(D) * N : 0;
+  return z;
+}
+
+// Every scratch buffer a reinitialization call of this kind uses, allocated
+// up front in the stencil (a recorded graph must not allocate).
+template <int D, class R>
+ReinitBufs<R> reinit_scratch(casl_stencil *st, bool so, bool tiled) {
+  const ReinitSizes z = reinit_sizes<D, R>(size_t(st->N), so, tiled);
+  ReinitBufs<R> b;
+  st->phi0.ensure(z.phi0);
+  st->flags.ensure(z.flags);
+  if (z.dd0) st->dd0.ensure(z.dd0);
+  b.phi0 = st->phi0.p;
+  b.flags = st->flags.p;
+  b.dd0 = st->dd0.p;
   if constexpr (sizeof(R) == 8) {
-    st->sfix64.ensure(size_t(F) * N);
-    st->tmp64.ensure(N);
-    if (so) st->ddn64.ensure(size_t(D) * N);
+    st->sfix64.ensure(z.sfix);
+    st->tmp64.ensure(z.tmp);
+    if (z.ddn) st->ddn64.ensure(z.ddn);
+    b.sfix = st->sfix64.p;
+    b.tmp = st->tmp64.p;
+    b.ddn = st->ddn64.p;
   } else {
-    st->sfixf.ensure(size_t(F) * N);
-    st->phif.ensure(N);
-    st->tmpf.ensure(N);
-    if (so) st->ddnf.ensure(size_t(D) * N);
+    st->sfixf.ensure(z.sfix);
+    st->phif.ensure(z.work);
+    st->tmpf.ensure(z.tmp);
+    if (z.ddn) st->ddnf.ensure(z.ddn);
+    b.sfix = st->sfixf.p;
+    b.work = st->phif.p;
+    b.tmp = st->tmpf.p;
+    b.ddn = st->ddnf.p;
   }
+  return b;
 }
 
 template <int D, class R>
 void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_space, int order_time, int n_iter,
-                double limit_high, double limit_low, hipStream_t s) {
+                double limit_high, double limit_low, hipStream_t s, const ReinitBufs<R> &B) {
   const int64_t N = st->N;
   const Tables<D> t = st->tables<D>();
   double lim_low, lim_high;
   widened_limits(limit_high, limit_low, st->zdt, lim_low, lim_high);
   const bool so = order_space > 1;
-  constexpr int F = 2 * D;
   const unsigned grid = grid_for(N);
 
   const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
-  reinit_scratch<D, R>(st, so, tiled);
-  HIP_CHECK(hipMemcpyAsync(st->phi0.p, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
+  HIP_CHECK(hipMemcpyAsync(B.phi0, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   if (so && !tiled) {
     prof_begin("k_dd", s);
-    hipLaunchKernelGGL((k_dd<D, double>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p);
+    hipLaunchKernelGGL((k_dd<D, double>), dim3(grid), dim3(BLOCK), 0, s, t, B.phi0, B.dd0);
     prof_end("k_dd", s);
     HIP_CHECK(hipGetLastError());
   }
-  R *sfix, *cur, *nxt, *ddn;
+  R *sfix = B.sfix, *cur, *nxt = B.tmp, *ddn = B.ddn;
   if constexpr (sizeof(R) == 8) {
-    sfix = st->sfix64.p;
     cur = phi;
-    nxt = st->tmp64.p;
-    ddn = st->ddn64.p;
   } else {
-    sfix = st->sfixf.p;
-    cur = st->phif.p;
-    nxt = st->tmpf.p;
-    ddn = st->ddnf.p;
-    hipLaunchKernelGGL(k_d2f, dim3(grid), dim3(BLOCK), 0, s, phi, st->phif.p, N);
+    cur = B.work;
+    hipLaunchKernelGGL(k_d2f, dim3(grid), dim3(BLOCK), 0, s, phi, B.work, N);
     HIP_CHECK(hipGetLastError());
   }
   if (tiled) {
     // flags and s_fix in tile-record order, the order the tiled stages read them
     TileArgs<double> tp = tile_args<double>(st);
-    tp.phin = st->phi0.p;
+    tp.phin = B.phi0;
     const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK_F64);
     prof_begin("k_tile_prep", s);
     if (so)
-      hipLaunchKernelGGL((k_tile_prep<D, R, true>), g, b, 0, s, tp, mask, lim_low, lim_high, st->zdt, st->flags.p,
+      hipLaunchKernelGGL((k_tile_prep<D, R, true>), g, b, 0, s, tp, mask, lim_low, lim_high, st->zdt, B.flags,
                          sfix);
     else
-      hipLaunchKernelGGL((k_tile_prep<D, R, false>), g, b, 0, s, tp, mask, lim_low, lim_high, st->zdt, st->flags.p,
+      hipLaunchKernelGGL((k_tile_prep<D, R, false>), g, b, 0, s, tp, mask, lim_low, lim_high, st->zdt, B.flags,
                          sfix);
     prof_end("k_tile_prep", s);
   } else {
     prof_begin("k_prep", s);
-    hipLaunchKernelGGL((k_prep<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, st->phi0.p, st->dd0.p, mask, lim_low,
-                       lim_high, st->zdt, int(so), st->flags.p, sfix);
+    hipLaunchKernelGGL((k_prep<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, B.phi0, B.dd0, mask, lim_low,
+                       lim_high, st->zdt, int(so), B.flags, sfix);
     prof_end("k_prep", s);
   }
   HIP_CHECK(hipGetLastError());
@@ -244,18 +315,18 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   auto stage_launch = [&](const R *in, R *out, bool avg) {
     if (so) {
       if (avg)
-        hipLaunchKernelGGL((k_stage<D, R, true, true>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
-                           st->phi0.p, in, ddn, out);
+        hipLaunchKernelGGL((k_stage<D, R, true, true>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0, in,
+                           ddn, out);
       else
-        hipLaunchKernelGGL((k_stage<D, R, true, false>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
-                           st->phi0.p, in, ddn, out);
+        hipLaunchKernelGGL((k_stage<D, R, true, false>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0,
+                           in, ddn, out);
     } else {
       if (avg)
-        hipLaunchKernelGGL((k_stage<D, R, false, true>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
-                           st->phi0.p, in, ddn, out);
+        hipLaunchKernelGGL((k_stage<D, R, false, true>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0,
+                           in, ddn, out);
       else
-        hipLaunchKernelGGL((k_stage<D, R, false, false>), dim3(grid), dim3(BLOCK), 0, s, t, st->flags.p, sfix,
-                           st->phi0.p, in, ddn, out);
+        hipLaunchKernelGGL((k_stage<D, R, false, false>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0,
+                           in, ddn, out);
     }
   };
   auto stage = [&](const R *in, R *out, bool avg) {
@@ -266,9 +337,9 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   TileArgs<R> ta{};
   if (tiled) {
     ta = tile_args<R>(st);
-    ta.flags = st->flags.p;
+    ta.flags = B.flags;
     ta.sfix = sfix;
-    ta.phi0 = st->phi0.p;
+    ta.phi0 = B.phi0;
   }
   auto tile_stage = [&](const R *in, R *out, bool avg) {
     ta.phin = in;
@@ -311,7 +382,11 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
   if constexpr (sizeof(R) == 8) {
     if (cur != phi) HIP_CHECK(hipMemcpyAsync(phi, cur, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   } else {
-    hipLaunchKernelGGL(k_f2d, dim3(grid), dim3(BLOCK), 0, s, cur, phi, N);
+    if (tiled)
+      hipLaunchKernelGGL(k_tile_f2d_keep<D>, dim3(unsigned(st->n_tiles)), dim3(BLOCK), 0, s, tile_args<float>(st),
+                         cur, B.flags, B.phi0, phi);
+    else
+      hipLaunchKernelGGL(k_f2d_keep, dim3(grid), dim3(BLOCK), 0, s, cur, B.flags, B.phi0, phi, N);
     HIP_CHECK(hipGetLastError());
   }
 }
@@ -330,11 +405,12 @@ namespace casl {
 
 // Independent bounds check of every record of a tiled encoding before
 // anything reaches the GPU: tiles within their LDS limits, halo nodes in
-// range, extension ranges matching the irregular records, every slot used
-// once, every corner slot readable, hanging faces valid.
+// range, record and extension ranges matching the per-tile counts, every slot
+// used once, every corner slot readable, hanging faces valid.
 void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
   const int F = 2 * D, C = 1 << (D - 1);
   const int WORDS = D == 2 ? HangExt<2>::WORDS : HangExt<3>::WORDS, SW = D == 2 ? HangExt<2>::SW : HangExt<3>::SW;
+  const int UW = D == 2 ? UniRec<2>::WORDS : UniRec<3>::WORDS;
   const int LMAX = D == 2 ? TileLimits<2>::LMAX : TileLimits<3>::LMAX;
   const int DMAX = D == 2 ? TileLimits<2>::DMAX : TileLimits<3>::DMAX;
   const int64_t nt = int64_t(H.tile_start.size()) - 1;
@@ -343,40 +419,65 @@ void validate_tile_host(const TileHost &H, int D, int64_t n_nodes) {
   if (int64_t(H.ext_start.size()) != nt + 1 || int64_t(H.nreg.size()) != nt || H.ext_start.front() != 0 ||
       size_t(H.ext_start.back()) * size_t(WORDS) != H.ext.size())
     throw std::runtime_error("extension ranges do not match the tiles");
+  if (int64_t(H.urec_start.size()) != nt + 1 || int64_t(H.nuni.size()) != nt || H.urec_start.front() != 0 ||
+      size_t(H.urec_start.back()) * size_t(UW) != H.urec.size() || H.rec_slot.size() != H.rec.size())
+    throw std::runtime_error("record ranges do not match the tiles");
   for (int64_t t = 0; t < nt; ++t) {
     const int nT = H.tile_start[t + 1] - H.tile_start[t];
     const int nH = H.halo_start[t + 1] - H.halo_start[t];
     const int nH1 = H.h1count[t];
     const int nR = H.rec_start[t + 1] - H.rec_start[t];
-    if (nT <= 0 || nH1 < 0 || nH1 > nH || nT + nH > LMAX || nT + nH1 > DMAX || nR != nT + nH1)
-      throw std::runtime_error("tile exceeds its LDS limits");
+    const int nU = H.urec_start[t + 1] - H.urec_start[t];
+    const int nTu = int(H.nuni[t] & 0xffffu), nH1u = int(H.nuni[t] >> 16);
+    if (nT <= 0 || nH1 < 0 || nH1 > nH || nT + nH > LMAX || nT + nH1 > DMAX || nR + nU != nT + nH1 ||
+        nTu > nT || nH1u > nH1 || nTu + nH1u != nU)
+      throw std::runtime_error("tile exceeds its LDS limits or its record counts disagree");
     for (int k = 0; k < nH; ++k) {
       const int32_t g = H.halo_idx[size_t(H.halo_start[t]) + k];
       if (g < 0 || g >= n_nodes) throw std::runtime_error("halo node out of range");
     }
-    // regular records first within T and within H1; the extension records of
-    // the irregular ones, in record order, fill [ext_start[t], ext_start[t+1])
+    // non-uniform records: regular first within T and within H1; the extension
+    // records of the irregular ones, in record order, fill [ext_start[t], ext_start[t+1])
+    const int nTn = nT - nTu, nH1n = nH1 - nH1u;  // non-uniform records of T and of H1
     const int nTreg = int(H.nreg[t] & 0xffffu), nH1reg = int(H.nreg[t] >> 16);
-    if (nTreg > nT || nH1reg > nH1 || H.ext_start[t + 1] - H.ext_start[t] != (nT - nTreg) + (nH1 - nH1reg))
+    if (nTreg > nTn || nH1reg > nH1n || H.ext_start[t + 1] - H.ext_start[t] != (nTn - nTreg) + (nH1n - nH1reg))
       throw std::runtime_error("regular record counts do not match the extension range");
-    std::vector<char> seen(size_t(nR), 0);
+    std::vector<char> seen(size_t(nT + nH1), 0);
+    // the tile's own records hold slots < nT, its H1 records slots nT .. nT + nH1 - 1, each exactly once
+    auto take_slot = [&](int slot, bool own) {
+      if (own ? (slot < 0 || slot >= nT) : (slot < nT || slot >= nT + nH1))
+        throw std::runtime_error("record slot out of range");
+      if (seen[size_t(slot)]++) throw std::runtime_error("record slot repeated");
+    };
+    for (int j = 0; j < nU; ++j) {
+      const uint32_t *w = &H.urec[(size_t(H.urec_start[t]) + size_t(j)) * size_t(UW)];
+      uint16_t idx[6];
+      int slot = 0, level = 0;
+      if (D == 2)
+        uni_unpack<2>(w, idx, slot, level);
+      else
+        uni_unpack<3>(w, idx, slot, level);
+      const bool own = j < nTu;
+      take_slot(slot, own);
+      // pass A of any record reads phi slots < nT + nH; pass B of a tile node
+      // also reads second-derivative slots < nT + nH1 at the same corners
+      const int bound = own ? nT + nH1 : nT + nH;
+      for (int f = 0; f < F; ++f)
+        if (idx[f] >= bound) throw std::runtime_error("corner slot out of range");
+    }
     for (int j = 0; j < nR; ++j) {
       const TileRec &rr = H.rec[size_t(H.rec_start[t]) + j];
       const int slot = H.rec_slot[size_t(H.rec_start[t]) + j];
-      // the first nT records are the tile's own nodes (slots < nT), the rest
-      // its H1 nodes (slots nT .. nT + nH1 - 1), each slot exactly once
-      if ((j < nT) ? (slot >= nT) : (slot < nT || slot >= nT + nH1)) throw std::runtime_error("record slot out of range");
-      if (seen[size_t(slot)]++) throw std::runtime_error("record slot repeated");
+      const bool own = j < nTn;
+      take_slot(slot, own);
       const bool irregular = (rr.geo & GEO_IRREGULAR) != 0;
-      if (irregular != ((j < nT) ? (j >= nTreg) : (j >= nT + nH1reg)))
+      if (irregular != (own ? (j >= nTreg) : (j >= nTn + nH1reg)))
         throw std::runtime_error("record out of its regular/irregular group");
-      // pass A of any record reads phi slots < nT + nH; pass B of a tile
-      // node also reads second-derivative slots < nT + nH1 at the same corners
-      const int bound = (j < nT) ? nT + nH1 : nT + nH;
+      const int bound = own ? nT + nH1 : nT + nH;
       for (int f = 0; f < F; ++f)
         if (rr.idx[f] >= bound) throw std::runtime_error("corner slot out of range");
       if (irregular) {
-        const int rank = (j < nT) ? j - nTreg : (nT - nTreg) + (j - nT - nH1reg);
+        const int rank = own ? j - nTreg : (nTn - nTreg) + (j - nTn - nH1reg);
         const uint16_t *e = &H.ext[(size_t(H.ext_start[t]) + size_t(rank)) * size_t(WORDS)];
         const int h0 = D == 2 ? e[0] : (e[0] & 15), h1 = D == 2 ? int(HANG_NONE) : ((e[0] >> 4) & 15);
         if (h0 >= F || (h1 != HANG_NONE && (h1 >= F || h1 <= h0))) throw std::runtime_error("bad hanging face");
@@ -401,15 +502,20 @@ void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st) {
   up(st->t_h1count, H.h1count);
   up(st->t_halo, H.halo_idx.empty() ? std::vector<int32_t>(1, 0) : H.halo_idx);
   up(st->t_rec_start, H.rec_start);
-  up(st->t_rec_slot, H.rec_slot);
-  st->t_rec.ensure(H.rec.size());
-  HIP_CHECK(hipMemcpy(st->t_rec.p, H.rec.data(), H.rec.size() * sizeof(TileRec), hipMemcpyHostToDevice));
+  up(st->t_rec_slot, H.rec_slot.empty() ? std::vector<uint16_t>(1, 0) : H.rec_slot);
+  st->t_rec.ensure(std::max<size_t>(H.rec.size(), 1));
+  if (!H.rec.empty())
+    HIP_CHECK(hipMemcpy(st->t_rec.p, H.rec.data(), H.rec.size() * sizeof(TileRec), hipMemcpyHostToDevice));
   up(st->t_ext, H.ext.empty() ? std::vector<uint16_t>(16, 0) : H.ext);
   up(st->t_ext_start, H.ext_start);
   up(st->t_nreg, H.nreg);
+  up(st->t_urec, H.urec.empty() ? std::vector<uint32_t>(4, 0) : H.urec);
+  up(st->t_urec_start, H.urec_start);
+  up(st->t_nuni, H.nuni);
   st->n_tiles = int64_t(H.tile_start.size()) - 1;
   st->n_halo = int64_t(H.halo_idx.size());
-  st->n_records = int64_t(H.rec.size());
+  st->n_records = int64_t(H.rec.size()) + int64_t(H.urec_start.back());
+  st->n_uniform = int64_t(H.urec_start.back());
   st->n_irregular = H.n_irregular;
   for (int a = 0; a < 3; ++a) {
     st->geom_base[a] = H.size[a];
@@ -619,14 +725,69 @@ int casl_tile_plan(const casl_flat_stencil *tables, int64_t *n_tiles, int64_t *n
     const int64_t nt = int64_t(H.tile_start.size()) - 1;
     const int W = tables->dim == 2 ? HangExt<2>::WORDS : HangExt<3>::WORDS;
     int64_t irr_nodes = 0, two = 0;
-    for (int64_t t = 0; t < nt; ++t) irr_nodes += (H.tile_start[t + 1] - H.tile_start[t]) - int64_t(H.nreg[t] & 0xffffu);
+    for (int64_t t = 0; t < nt; ++t)
+      irr_nodes += (H.tile_start[t + 1] - H.tile_start[t]) - int64_t(H.nuni[t] & 0xffffu) - int64_t(H.nreg[t] & 0xffffu);
     if (tables->dim == 3)
       for (int64_t i = 0; i < H.n_irregular; ++i) two += ((H.ext[size_t(i) * W] >> 4) & 15) != HANG_NONE;
     if (n_tiles) *n_tiles = nt;
-    if (n_records) *n_records = int64_t(H.rec.size());
+    if (n_records) *n_records = int64_t(H.rec.size()) + int64_t(H.urec_start.back());
     if (n_irregular) *n_irregular = H.n_irregular;
     if (n_irregular_nodes) *n_irregular_nodes = irr_nodes;
     if (n_two_hanging) *n_two_hanging = two;
+  });
+}
+
+int casl_stencil_tile_check(const casl_stencil_t *st, const casl_flat_stencil *tables, int32_t *equal) {
+  return guarded([&]() {
+    if (!st || !tables || !equal) throw std::invalid_argument("null argument");
+    validate_tables(tables);
+    if (tables->n_nodes != st->N || tables->dim != st->dim) throw std::invalid_argument("tables of another grid");
+    *equal = 0;
+    TileHost H;
+    try {
+      build_tiles(*tables, H);
+    } catch (const std::exception &e) {
+      // neither encoding exists: equal
+      if (!st->tiled) *equal = 1;
+      else set_error(std::string("the host builder refuses these tables: ") + e.what());
+      return;
+    }
+    if (!st->tiled) {
+      set_error("the stencil has no tiled encoding: " + st->tile_error);
+      return;
+    }
+    validate_tile_host(H, tables->dim, tables->n_nodes);
+    auto same = [&](const char *name, const void *dev, const void *host, size_t bytes) {
+      std::vector<unsigned char> d(bytes);
+      if (bytes) HIP_CHECK(hipMemcpy(d.data(), dev, bytes, hipMemcpyDeviceToHost));
+      if (bytes && std::memcmp(d.data(), host, bytes) != 0) {
+        set_error(std::string("tiled encodings differ in ") + name);
+        return false;
+      }
+      return true;
+    };
+    const int64_t nt = int64_t(H.tile_start.size()) - 1;
+    if (st->n_tiles != nt || st->n_halo != int64_t(H.halo_idx.size()) ||
+        st->n_records != int64_t(H.rec.size()) + int64_t(H.urec_start.back()) || st->n_irregular != H.n_irregular ||
+        st->geom_base[0] != H.size[0] || st->geom_unit[0] != H.size[0] * std::ldexp(1.0, -H.lmax)) {
+      set_error("tiled encodings differ in their counts or scales");
+      return;
+    }
+    const bool ok =
+        same("tile_start", st->t_tile_start.p, H.tile_start.data(), H.tile_start.size() * sizeof(int32_t)) &&
+        same("halo_start", st->t_halo_start.p, H.halo_start.data(), H.halo_start.size() * sizeof(int32_t)) &&
+        same("h1count", st->t_h1count.p, H.h1count.data(), H.h1count.size() * sizeof(int32_t)) &&
+        same("halo", st->t_halo.p, H.halo_idx.data(), H.halo_idx.size() * sizeof(int32_t)) &&
+        same("rec_start", st->t_rec_start.p, H.rec_start.data(), H.rec_start.size() * sizeof(int32_t)) &&
+        same("rec", st->t_rec.p, H.rec.data(), H.rec.size() * sizeof(TileRec)) &&
+        same("rec_slot", st->t_rec_slot.p, H.rec_slot.data(), H.rec_slot.size() * sizeof(uint16_t)) &&
+        same("ext", st->t_ext.p, H.ext.data(), H.ext.size() * sizeof(uint16_t)) &&
+        same("ext_start", st->t_ext_start.p, H.ext_start.data(), H.ext_start.size() * sizeof(int32_t)) &&
+        same("nreg", st->t_nreg.p, H.nreg.data(), H.nreg.size() * sizeof(uint32_t)) &&
+        same("urec", st->t_urec.p, H.urec.data(), H.urec.size() * sizeof(uint32_t)) &&
+        same("urec_start", st->t_urec_start.p, H.urec_start.data(), H.urec_start.size() * sizeof(int32_t)) &&
+        same("nuni", st->t_nuni.p, H.nuni.data(), H.nuni.size() * sizeof(uint32_t));
+    *equal = ok ? 1 : 0;
   });
 }
 
@@ -650,6 +811,8 @@ int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd, vo
   });
 }
 
+}  // extern "C"
+
 static void check_reinit_args(casl_stencil_t *st, double *phi, int order_space, int order_time, int n_iter,
                               double limit_high, double limit_low, int precision) {
   if (!st || !phi) throw std::invalid_argument("null argument");
@@ -662,30 +825,214 @@ static void check_reinit_args(casl_stencil_t *st, double *phi, int order_space, 
     throw std::invalid_argument("unknown precision mode");
 }
 
-// prepare_only: allocate the call's scratch, launch nothing
+// One call on the stencil's own scratch buffers (allocated on first use);
+// prepare_only: allocate them, launch nothing.
+template <int D, class R>
+static void reinit_on_stencil(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space,
+                              int order_time, int n_iter, double limit_high, double limit_low, hipStream_t s,
+                              bool prepare_only) {
+  const bool so = order_space > 1, tiled = st->tiled && st->path == CASL_PATH_TILED;
+  const ReinitBufs<R> B = reinit_scratch<D, R>(st, so, tiled);
+  if (!prepare_only) run_reinit<D, R>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s, B);
+}
+
 static void dispatch_reinit(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space, int order_time,
                             int n_iter, double limit_high, double limit_low, int precision, hipStream_t s,
                             bool prepare_only = false) {
-  const bool so = order_space > 1, tiled = st->tiled && st->path == CASL_PATH_TILED;
   const bool fp64 = precision == CASL_PRECISION_FP64;
-  if (prepare_only) {
-    if (st->dim == 2)
-      fp64 ? reinit_scratch<2, double>(st, so, tiled) : reinit_scratch<2, float>(st, so, tiled);
-    else
-      fp64 ? reinit_scratch<3, double>(st, so, tiled) : reinit_scratch<3, float>(st, so, tiled);
-    return;
-  }
   if (st->dim == 2) {
     if (fp64)
-      run_reinit<2, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+      reinit_on_stencil<2, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s,
+                                   prepare_only);
     else
-      run_reinit<2, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+      reinit_on_stencil<2, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s,
+                                  prepare_only);
   } else {
     if (fp64)
-      run_reinit<3, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+      reinit_on_stencil<3, double>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s,
+                                   prepare_only);
     else
-      run_reinit<3, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s);
+      reinit_on_stencil<3, float>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s,
+                                  prepare_only);
   }
+}
+
+// One call with its buffers carved from [base, base + cap) instead of the
+// stencil's own scratch: nothing is allocated.  The buffers are 256-byte
+// aligned pieces in the order of ReinitBufs.
+template <int D, class R>
+static void reinit_in_buffer_t(casl_stencil_t *st, double *phi, int order_space, int order_time, int n_iter,
+                               double limit_high, double limit_low, hipStream_t s, unsigned char *base, size_t cap) {
+  const bool so = order_space > 1, tiled = st->tiled && st->path == CASL_PATH_TILED;
+  const ReinitSizes z = reinit_sizes<D, R>(size_t(st->N), so, tiled);
+  const size_t bytes[7] = {z.phi0 * sizeof(double), z.flags * sizeof(uint16_t), z.dd0 * sizeof(double),
+                           z.sfix * sizeof(R), z.work * sizeof(R), z.tmp * sizeof(R), z.ddn * sizeof(R)};
+  size_t total = 0;
+  for (size_t b : bytes) total += (b + 255) & ~size_t(255);
+  if (total > cap)
+    throw std::invalid_argument("reinitialization of " + std::to_string(st->N) + " nodes needs " +
+                                std::to_string(total) + " bytes of scratch, more than the fixed " +
+                                std::to_string(cap) + " bytes");
+  unsigned char *p[7];
+  size_t off = 0;
+  for (int i = 0; i < 7; ++i) {
+    p[i] = bytes[i] ? base + off : nullptr;
+    off += (bytes[i] + 255) & ~size_t(255);
+  }
+  ReinitBufs<R> B;
+  B.phi0 = reinterpret_cast<double *>(p[0]);
+  B.flags = reinterpret_cast<uint16_t *>(p[1]);
+  B.dd0 = reinterpret_cast<double *>(p[2]);
+  B.sfix = reinterpret_cast<R *>(p[3]);
+  B.work = reinterpret_cast<R *>(p[4]);
+  B.tmp = reinterpret_cast<R *>(p[5]);
+  B.ddn = reinterpret_cast<R *>(p[6]);
+  run_reinit<D, R>(st, phi, nullptr, order_space, order_time, n_iter, limit_high, limit_low, s, B);
+}
+
+namespace casl {
+void reinit_in_buffer(casl_stencil_t *st, double *phi, int order_space, int order_time, int n_iter,
+                      double limit_high, double limit_low, int precision, hipStream_t s, unsigned char *base,
+                      size_t cap) {
+  check_reinit_args(st, phi, order_space, order_time, n_iter, limit_high, limit_low, precision);
+  if (!base) throw std::invalid_argument("null scratch");
+  // no timing events: creating them is the kind of call this path avoids
+  const bool rec = g_recording;
+  g_recording = true;
+  try {
+    const bool fp64 = precision == CASL_PRECISION_FP64;
+    if (st->dim == 2)
+      fp64 ? reinit_in_buffer_t<2, double>(st, phi, order_space, order_time, n_iter, limit_high, limit_low, s, base,
+                                           cap)
+           : reinit_in_buffer_t<2, float>(st, phi, order_space, order_time, n_iter, limit_high, limit_low, s, base,
+                                          cap);
+    else
+      fp64 ? reinit_in_buffer_t<3, double>(st, phi, order_space, order_time, n_iter, limit_high, limit_low, s, base,
+                                           cap)
+           : reinit_in_buffer_t<3, float>(st, phi, order_space, order_time, n_iter, limit_high, limit_low, s, base,
+                                          cap);
+  } catch (...) {
+    g_recording = rec;
+    throw;
+  }
+  g_recording = rec;
+}
+}  // namespace casl
+
+// ------------------------------------------------------ host-buffer calls
+// phi in host memory (a PETSc Vec's array) reaches the device through the
+// stencil's pinned staging buffer in chunks: the host threads copy chunk k
+// into pinned memory while the DMA engine moves chunk k - 1 to the device,
+// and on the way back each chunk is copied out as soon as its D2H transfer
+// has finished.  Copies from pageable memory would go through the driver's
+// own bounce buffer one piece at a time.
+namespace {
+
+int stage_chunks(int64_t N) { return int(std::max<int64_t>(1, std::min<int64_t>(16, N >> 20))); }
+
+void chunk_range(int64_t N, int K, int k, int64_t &b, int64_t &e) {
+  b = N * k / K;
+  e = N * (k + 1) / K;
+}
+
+// pageable src -> pinned stage -> device dst, chunk by chunk on stream s
+void stage_in(casl_stencil *st, const double *src, double *dst, int64_t N, hipStream_t s) {
+  const int K = stage_chunks(N), T = std::max(1, std::min(num_threads(), 16));
+  double *stage = st->host_stage.p;
+  std::vector<std::atomic<int>> done(static_cast<size_t>(K));
+  for (auto &d : done) d.store(0);
+  std::vector<std::thread> pool;
+  pool.reserve(size_t(T));
+  for (int t = 0; t < T; ++t)
+    pool.emplace_back([&, t]() {
+      for (int k = 0; k < K; ++k) {
+        int64_t b, e;
+        chunk_range(N, K, k, b, e);
+        const int64_t pb = b + (e - b) * t / T, pe = b + (e - b) * (t + 1) / T;
+        if (pe > pb) std::memcpy(stage + pb, src + pb, size_t(pe - pb) * sizeof(double));
+        done[size_t(k)].fetch_add(1, std::memory_order_release);
+      }
+    });
+  hipError_t err = hipSuccess;
+  for (int k = 0; k < K; ++k) {
+    while (done[size_t(k)].load(std::memory_order_acquire) < T) std::this_thread::yield();
+    int64_t b, e;
+    chunk_range(N, K, k, b, e);
+    if (err == hipSuccess && e > b)
+      err = hipMemcpyAsync(dst + b, stage + b, size_t(e - b) * sizeof(double), hipMemcpyHostToDevice, s);
+  }
+  for (auto &th : pool) th.join();
+  if (err != hipSuccess) throw hip_error(std::string("staging copy to the device failed: ") + hipGetErrorString(err));
+}
+
+// device src -> pinned stage -> pageable dst; returns when dst is written
+void stage_out(casl_stencil *st, const double *src, double *dst, int64_t N, hipStream_t s) {
+  const int K = stage_chunks(N), T = std::max(1, std::min(num_threads(), 16));
+  double *stage = st->host_stage.p;
+  for (int k = 0; k < K; ++k) {
+    int64_t b, e;
+    chunk_range(N, K, k, b, e);
+    HIP_CHECK(hipMemcpyAsync(stage + b, src + b, size_t(e - b) * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipEventRecord(st->stage_events[size_t(k)], s));
+  }
+  std::atomic<int> bad{0};
+  std::vector<std::thread> pool;
+  pool.reserve(size_t(T));
+  for (int t = 0; t < T; ++t)
+    pool.emplace_back([&, t]() {
+      for (int k = 0; k < K; ++k) {
+        if (hipEventSynchronize(st->stage_events[size_t(k)]) != hipSuccess) {
+          bad.store(1);
+          return;
+        }
+        int64_t b, e;
+        chunk_range(N, K, k, b, e);
+        const int64_t pb = b + (e - b) * t / T, pe = b + (e - b) * (t + 1) / T;
+        if (pe > pb) std::memcpy(dst + pb, stage + pb, size_t(pe - pb) * sizeof(double));
+      }
+    });
+  for (auto &th : pool) th.join();
+  if (bad.load()) throw hip_error("staging copy from the device failed");
+}
+
+void ensure_staging(casl_stencil *st) {
+  const size_t N = size_t(st->N);
+  st->host_phi.ensure(N);
+  st->host_stage.ensure(N);
+  const size_t K = size_t(stage_chunks(st->N));
+  while (st->stage_events.size() < K) {
+    hipEvent_t e;
+    HIP_CHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    st->stage_events.push_back(e);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int casl_reinit_host(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space, int order_pseudotime,
+                     int n_iter, double limit_high, double limit_low, int precision, void *stream) {
+  return guarded([&]() {
+    check_reinit_args(st, phi, order_space, order_pseudotime, n_iter, limit_high, limit_low, precision);
+    if (mask && (order_space != 2 || order_pseudotime != 2))
+      throw std::invalid_argument("reinitialize_2nd_order_with_mask: a mask comes with orders 2 and 2 only");
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    ensure_staging(st);
+    const uint8_t *dmask = nullptr;
+    if (mask) {
+      st->host_mask.ensure(size_t(st->N));
+      st->host_mask_stage.ensure(size_t(st->N));
+      std::memcpy(st->host_mask_stage.p, mask, size_t(st->N));
+      HIP_CHECK(hipMemcpyAsync(st->host_mask.p, st->host_mask_stage.p, size_t(st->N), hipMemcpyHostToDevice, s));
+      dmask = st->host_mask.p;
+    }
+    stage_in(st, phi, st->host_phi.p, st->N, s);
+    dispatch_reinit(st, st->host_phi.p, dmask, order_space, order_pseudotime, n_iter, limit_high, limit_low,
+                    precision, s);
+    HIP_CHECK(hipGetLastError());
+    stage_out(st, st->host_phi.p, phi, st->N, s);
+  });
 }
 
 static int reinit_common(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space, int order_time,

@@ -58,6 +58,34 @@ struct DevBuf {
   }
 };
 
+// Page-locked host memory (hipHostMalloc), grown on demand.
+template <class T>
+struct PinnedBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf &) = delete;
+  PinnedBuf &operator=(const PinnedBuf &) = delete;
+  ~PinnedBuf() { release(); }
+  void release() {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void ensure(size_t count) {
+    if (n >= count && p) return;
+    release();
+    if (count == 0) return;
+    hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) {
+      p = nullptr;
+      if (e == hipErrorOutOfMemory) throw std::bad_alloc();
+      throw hip_error(std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+    }
+    n = count;
+  }
+};
+
 // Maps exceptions to the C ABI status codes.
 template <class Fn>
 int guarded(Fn fn) {
@@ -391,7 +419,7 @@ struct casl_stencil {
   // tiled encoding (tiles.hpp)
   bool tiled = false;
   int path = CASL_PATH_TILED;
-  int64_t n_tiles = 0, n_halo = 0, n_records = 0, n_irregular = 0;
+  int64_t n_tiles = 0, n_halo = 0, n_records = 0, n_irregular = 0, n_uniform = 0;
   double geom_base[3] = {0, 0, 0}, geom_unit[3] = {0, 0, 0};
   std::string tile_error;
   casl::DevBuf<int32_t> t_tile_start, t_halo_start, t_h1count, t_halo, t_rec_start;
@@ -400,6 +428,16 @@ struct casl_stencil {
   casl::DevBuf<uint16_t> t_ext;
   casl::DevBuf<int32_t> t_ext_start;
   casl::DevBuf<uint32_t> t_nreg;
+  casl::DevBuf<uint32_t> t_urec;
+  casl::DevBuf<int32_t> t_urec_start;
+  casl::DevBuf<uint32_t> t_nuni;
+  // casl_reinit_host: phi on the device and its pinned staging copy, and one
+  // event per staging chunk
+  casl::DevBuf<double> host_phi;
+  casl::PinnedBuf<double> host_stage;
+  casl::DevBuf<uint8_t> host_mask;
+  casl::PinnedBuf<uint8_t> host_mask_stage;
+  std::vector<hipEvent_t> stage_events;
   // casl_reinit_graph: the recorded call and what it was recorded for
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
@@ -408,7 +446,10 @@ struct casl_stencil {
   casl_stencil() = default;
   casl_stencil(const casl_stencil &) = delete;
   casl_stencil &operator=(const casl_stencil &) = delete;
-  ~casl_stencil() { drop_graph(); }
+  ~casl_stencil() {
+    drop_graph();
+    for (hipEvent_t e : stage_events) (void)hipEventDestroy(e);
+  }
   void drop_graph() {
     if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -436,6 +477,20 @@ void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_til
 // into st (tile arrays, one geometry).
 void validate_tile_host(const TileHost &H, int D, int64_t n_nodes);
 void upload_tile_host(const TileHost &H, int64_t n_nodes, casl_stencil *st);
+
+// Grid-wide tables of a stencil from tables already on the device (checked
+// there, distances converted to FP32 exactly), and the tiled encoding built
+// on the device (tiles_gpu.hip); on failure the latter leaves the stencil on
+// the two-pass path with the reason in tile_error.
+void upload_device_tables(const casl_flat_stencil &dtab, casl_stencil *st, hipStream_t s);
+void build_tiles_device(const casl_flat_stencil &dtab, casl_stencil *st, hipStream_t s);
+
+// A reinitialization call (casl_reinit's arguments, no mask) whose buffers are
+// carved from [base, base + cap) instead of the stencil's own scratch:
+// nothing is allocated.  Throws std::invalid_argument when they do not fit.
+void reinit_in_buffer(casl_stencil *st, double *phi, int order_space, int order_time, int n_iter,
+                      double limit_high, double limit_low, int precision, hipStream_t s, unsigned char *base,
+                      size_t cap);
 
 // Kernel timing (casl_profile_*): prof_begin/prof_end bracket one launch.
 bool prof_on();

@@ -21,6 +21,7 @@ struct Tables {
   static constexpr bool REGULAR = false;  // see the REGULAR shortcuts below
   static constexpr bool FASTDIV = false;  // see second_difference
   static constexpr bool SLOTS = false;    // see the hanging-face functions below
+  static constexpr bool UNIFORM = false;  // see second_difference
   static constexpr int F = 2 * D;
   static constexpr int C = 1 << (D - 1);
   static constexpr int T = 2 * (D - 1);
@@ -204,8 +205,16 @@ __device__ __forceinline__ R central_second_derivative(R fp, R f0, R fm, R dp, R
 // divisions by d_p and d_m become multiplications by their exact inverses and
 // the division by d_p + d_m one by 1/(2 d_p) when d_p == d_m: the same
 // correctly rounded results, without the division sequences.
+//
+// TB::UNIFORM: every arm has the same power-of-two length q (a uniform node of
+// the tiled encoding on a dyadic geometry).  Then (fp - f0) * (1/q) and
+// (fm - f0) * (1/q) are exact scalings, their correctly rounded sum is the
+// rounded sum of the differences times 1/q, and the factors 2 and 1/(2q)
+// are exact too: ((fp - f0) + (fm - f0)) * (1/q)^2 gives the same bits as the
+// path above, in FP64 and (with its fused first product) in FP32.
 template <class R, class TB>
 __device__ __forceinline__ R second_difference(const TB &t, int a, R fp, R f0, R fm, int64_t n) {
+  if constexpr (TB::UNIFORM) return ((fp - f0) + (fm - f0)) * t.iq2;
   const int fpp = 2 * a + 1, fmm = 2 * a;
   if constexpr (TB::FASTDIV) {
     if (t.fast) {

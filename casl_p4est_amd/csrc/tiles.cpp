@@ -119,8 +119,19 @@ struct Builder {
     std::vector<TileRec> rec;
     std::vector<uint16_t> slot;
     std::vector<uint16_t> ext;  // extension records of this tile's irregular records, in record order
-    int n_treg = 0, n_h1reg = 0;  // regular records of T and of H1
+    std::vector<uint32_t> urec;   // uniform records (UniRec<D>), T then H1
+    int n_treg = 0, n_h1reg = 0;  // regular (non-uniform) records of T and of H1
+    int n_tuni = 0, n_h1uni = 0;  // uniform records of T and of H1
   };
+  static constexpr int UW = UniRec<D>::WORDS;
+
+  // a regular record off the walls whose arms all sit at one level
+  static bool uniform(const TileRec &r) {
+    if (r.geo & (GEO_IRREGULAR | (63u << 24))) return false;
+    for (int f = 1; f < F; ++f)
+      if (((r.geo >> (4 * f)) & 15u) != (r.geo & 15u)) return false;
+    return true;
+  }
   static constexpr int NH = HangExt<D>::NH, WORDS = HangExt<D>::WORDS, SW = HangExt<D>::SW;
 
   // local index of global node g in tile (s, e, h1, h2)
@@ -274,6 +285,7 @@ struct Builder {
     t.rec.clear();
     t.slot.clear();
     t.ext.clear();
+    t.urec.clear();
     const size_t nall = size_t(nT) + t.h1.size();
     std::vector<TileRec> all(nall);
     std::vector<uint16_t> exts(nall * WORDS);
@@ -286,21 +298,32 @@ struct Builder {
       const size_t j = size_t(nT) + i;
       all[j] = encode(t, t.h1[i], &exts[j * WORDS], &kind[j]);
     }
-    // group: T regular, T hanging on 1 .. NH faces, then the same for H1;
-    // extension records follow the irregular records in that order
+    // kind -1: uniform (compact record)
+    for (size_t j = 0; j < nall; ++j)
+      if (kind[j] == 0 && uniform(all[j])) kind[j] = -1;
+    // group: T uniform, T regular, T hanging on 1 .. NH faces, then the same
+    // for H1; extension records follow the irregular records in that order
     auto emit = [&](size_t lo, size_t hi, int k) {
       int cnt = 0;
       for (size_t j = lo; j < hi; ++j)
         if (kind[j] == k) {
-          t.rec.push_back(all[j]);
-          t.slot.push_back(uint16_t(j));
-          if (k > 0) t.ext.insert(t.ext.end(), exts.begin() + j * WORDS, exts.begin() + (j + 1) * WORDS);
+          if (k < 0) {
+            uint32_t w[UW];
+            uni_pack<D>(all[j].idx, uint32_t(j), all[j].geo & 15u, w);
+            t.urec.insert(t.urec.end(), w, w + UW);
+          } else {
+            t.rec.push_back(all[j]);
+            t.slot.push_back(uint16_t(j));
+            if (k > 0) t.ext.insert(t.ext.end(), exts.begin() + j * WORDS, exts.begin() + (j + 1) * WORDS);
+          }
           ++cnt;
         }
       return cnt;
     };
+    t.n_tuni = emit(0, size_t(nT), -1);
     t.n_treg = emit(0, size_t(nT), 0);
     for (int k = 1; k <= NH; ++k) emit(0, size_t(nT), k);
+    t.n_h1uni = emit(size_t(nT), nall, -1);
     t.n_h1reg = emit(size_t(nT), nall, 0);
     for (int k = 1; k <= NH; ++k) emit(size_t(nT), nall, k);
   }
@@ -354,6 +377,9 @@ struct Builder {
     H.tile_start.assign(1, 0);
     H.halo_start.assign(1, 0);
     H.rec_start.assign(1, 0);
+    H.urec_start.assign(1, 0);
+    H.urec.clear();
+    H.nuni.clear();
     H.h1count.clear();
     H.halo_idx.clear();
     H.rec.clear();
@@ -376,9 +402,13 @@ struct Builder {
         if (n_ext >= size_t(INT32_MAX)) throw std::runtime_error("too many irregular records");
         H.ext_start.push_back(int32_t(n_ext));
         H.nreg.push_back(uint32_t(t.n_treg) | (uint32_t(t.n_h1reg) << 16));
+        H.urec.insert(H.urec.end(), t.urec.begin(), t.urec.end());
+        H.urec_start.push_back(int32_t(H.urec.size() / UW));
+        H.nuni.push_back(uint32_t(t.n_tuni) | (uint32_t(t.n_h1uni) << 16));
       }
     H.n_irregular = int64_t(n_ext);
-    if (H.halo_idx.size() >= size_t(INT32_MAX) || H.rec.size() >= size_t(INT32_MAX))
+    if (H.halo_idx.size() >= size_t(INT32_MAX) || H.rec.size() >= size_t(INT32_MAX) ||
+        H.urec.size() / UW >= size_t(INT32_MAX))
       throw std::runtime_error("tile tables too large for int32 offsets");
   }
 };

@@ -75,6 +75,7 @@ template <int D, class R>
 struct DecodedDist {
   static constexpr bool FASTDIV = true;
   static constexpr bool SLOTS = false;
+  static constexpr bool UNIFORM = false;
   static constexpr int F = 2 * D;
   R dist[F];
   R q[F];
@@ -102,6 +103,54 @@ struct RegStencil : DecodedDist<D, R> {
     this->decode(r.geo, g);
   }
 };
+
+// Stencil of a uniform node on a dyadic geometry (every arm cell a power of
+// two; the kernels take this accessor only when TileGeom::pow2): one corner
+// per face, all arms q = base * 2^-level long, no wall.  The per-face arrays
+// hold the same value so the shared stencil code reads them as it reads a
+// RegStencil's.
+template <int D, class R>
+struct UniStencil {
+  static constexpr bool REGULAR = true;
+  static constexpr bool FASTDIV = true;
+  static constexpr bool SLOTS = false;
+  static constexpr bool UNIFORM = true;
+  static constexpr bool fast = true;
+  static constexpr int F = 2 * D;
+  uint16_t id[F];
+  R dist[F], q[F], iq[F];
+  R iq2;  // 1 / q^2
+  __device__ __forceinline__ int32_t idx(int f, int, int64_t) const { return id[f]; }
+  __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
+  __device__ __forceinline__ R dt(int f, int k, int64_t) const { return (k & 1) ? q[f] : R(0); }
+  // w: the UniRec<D> words; returns the record's LDS slot
+  __device__ __forceinline__ int load(const uint32_t *w, const TileGeom<R> &g) {
+    int slot, level;
+    uni_unpack<D>(w, id, slot, level);
+    const R qq = g.base[0] * pow2neg(R(0), level), iqq = g.inv_base[0] * pow2pos(R(0), level);
+#pragma unroll
+    for (int f = 0; f < F; ++f) {
+      dist[f] = qq;
+      q[f] = qq;
+      iq[f] = iqq;
+    }
+    iq2 = iqq * iqq;
+    return slot;
+  }
+};
+
+// The 16-byte record of a uniform node (the general decoding's view of it).
+template <int D>
+__device__ __forceinline__ TileRec uni_as_rec(const uint32_t *w, int &slot) {
+  TileRec r;
+  int level;
+  uni_unpack<D>(w, r.idx, slot, level);
+  uint32_t geo = 0;
+#pragma unroll
+  for (int f = 0; f < 2 * D; ++f) geo |= uint32_t(level) << (4 * f);
+  r.geo = geo;
+  return r;
+}
 
 // Stencil of a node with hanging arms: the record's one corner per face, and
 // for the faces that hang (at most D - 1) the extension record's corners and
@@ -229,7 +278,10 @@ struct TileArgs {
   const uint16_t *__restrict__ rec_slot;
   const uint16_t *__restrict__ ext;
   const int32_t *__restrict__ ext_start;  // [tiles + 1] first extension record of each tile
-  const uint32_t *__restrict__ nreg;      // [tiles] regular records of T | of H1 << 16
+  const uint32_t *__restrict__ nreg;      // [tiles] regular (non-uniform) records of T | of H1 << 16
+  const uint32_t *__restrict__ urec;      // uniform records (UniRec<D> words)
+  const int32_t *__restrict__ urec_start; // [tiles + 1] first uniform record of each tile
+  const uint32_t *__restrict__ nuni;      // [tiles] uniform records of T | of H1 << 16
   TileGeom<R> geom;
   const uint16_t *__restrict__ flags;
   const R *__restrict__ sfix;
@@ -238,6 +290,118 @@ struct TileArgs {
   R *out;
   int64_t N;
 };
+
+// The ranges of one tile and the map from a record position i (0 <= i <
+// nT + nH1: the tile's nodes, then its H1 nodes) to its record: uniform
+// records first within T and within H1 (urec), the others after them (rec,
+// rec_slot, and ext for those with hanging faces).
+template <int D>
+struct TileCtx {
+  static constexpr int UW = UniRec<D>::WORDS;
+  int s, nT, hb, nH, nH1, rb, ub, ext0, nTu, nH1u;
+  uint32_t nreg;
+  template <class R>
+  __device__ __forceinline__ void load(const TileArgs<R> &A, int tile) {
+    s = A.tile_start[tile];
+    nT = A.tile_start[tile + 1] - s;
+    hb = A.halo_start[tile];
+    nH = A.halo_start[tile + 1] - hb;
+    nH1 = A.h1count[tile];
+    rb = A.rec_start[tile];
+    ub = A.urec_start[tile];
+    ext0 = A.ext_start[tile];
+    nreg = A.nreg[tile];
+    const uint32_t nu = A.nuni[tile];
+    nTu = int(nu & 0xffffu);
+    nH1u = int(nu >> 16);
+  }
+  __device__ __forceinline__ bool uniform(int i) const { return i < nT ? i < nTu : (i - nT) < nH1u; }
+  // uniform position i: its record's first word
+  __device__ __forceinline__ int64_t uword(int i) const {
+    return (int64_t(ub) + (i < nT ? i : nTu + (i - nT))) * UW;
+  }
+  // other position i: its rank among the tile's non-uniform records
+  __device__ __forceinline__ int npos(int i) const { return i < nT ? i - nTu : (nT - nTu) + (i - nT - nH1u); }
+  __device__ __forceinline__ const uint16_t *ext_of(const uint16_t *__restrict__ ext, int i) const {
+    return hang_record(ext, ext0, nreg, nT - nTu, npos(i), HangExt<D>::WORDS);
+  }
+};
+
+// A record held in registers: the UniRec words of a uniform position, or the
+// 16-byte TileRec of any other (whose LDS slot the caller keeps beside it).
+struct RecRegs {
+  uint32_t w[4];
+};
+
+// Loads the record at position i; returns its LDS slot for a non-uniform
+// record (uniform records carry theirs: uni_slot).
+template <int D, class R>
+__device__ __forceinline__ int fetch_record(const TileArgs<R> &A, const TileCtx<D> &c, int i, RecRegs &r) {
+  if (c.uniform(i)) {
+    const uint32_t *p = A.urec + c.uword(i);
+#pragma unroll
+    for (int k = 0; k < UniRec<D>::WORDS; ++k) r.w[k] = p[k];
+    return 0;
+  }
+  const int j = c.rb + c.npos(i);
+  const uint4 v = reinterpret_cast<const uint4 *>(A.rec)[j];
+  r.w[0] = v.x;
+  r.w[1] = v.y;
+  r.w[2] = v.z;
+  r.w[3] = v.w;
+  return A.rec_slot[j];
+}
+
+// The general view (TileRec) of the record at position i; slot: the slot
+// fetch_record returned, replaced by the record's own for a uniform one.
+template <int D>
+__device__ __forceinline__ TileRec record_at(const TileCtx<D> &c, int i, const RecRegs &r, int &slot) {
+  if (c.uniform(i)) return uni_as_rec<D>(r.w, slot);
+  TileRec rec;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    rec.idx[2 * k] = uint16_t(r.w[k] & 0xffffu);
+    rec.idx[2 * k + 1] = uint16_t(r.w[k] >> 16);
+  }
+  rec.geo = r.w[3];
+  return rec;
+}
+
+// The LDS slot of the record at position i.
+template <int D>
+__device__ __forceinline__ int slot_at(const TileCtx<D> &c, int i, const RecRegs &r, int fetched_slot) {
+  return c.uniform(i) ? uni_slot<D>(r.w) : fetched_slot;
+}
+
+// Pass A (laplace) of the record at position i into dds[a * DMAX + slot].
+// Uniform records on a dyadic geometry (ufast) take UniStencil, the others
+// the general decoding.
+template <int D, class R>
+__device__ __forceinline__ void tile_pass_a(const TileArgs<R> &A, const TileCtx<D> &c, bool ufast, const RecRegs &r,
+                                            int fetched_slot, int i, const R *__restrict__ phis,
+                                            R *__restrict__ dds) {
+  constexpr int DMAX = TileLimits<D>::DMAX;
+  R f000, fnb[2 * D], dd[D];
+  int j = fetched_slot;
+  if (ufast && c.uniform(i)) {
+    UniStencil<D, R> st;
+    j = st.load(r.w, A.geom);
+    laplace<D, R>(st, j, phis, f000, fnb, dd);
+  } else {
+    const TileRec rec = record_at<D>(c, i, r, j);
+    if (!(rec.geo & GEO_IRREGULAR)) {
+      RegStencil<D, R> st;
+      st.load(rec, A.geom);
+      laplace<D, R>(st, j, phis, f000, fnb, dd);
+    } else {
+      HangStencil<D, R> st;
+      st.load(rec, c.ext_of(A.ext, i), A.geom);
+      laplace<D, R>(st, j, phis, f000, fnb, dd);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
+}
 
 // Threads per block of the tile kernels: 1024, two tile nodes per thread.
 // The FP64 kernels hold 160 KB of LDS arrays, so one block runs per compute
@@ -284,15 +448,12 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   constexpr int DMAX = TileLimits<D>::DMAX;
   constexpr int KT = TileLimits<D>::TMAX / BLOCK;  // tile nodes per thread, at most
   static_assert(TileLimits<D>::TMAX % BLOCK == 0, "tile size must be a multiple of the block");
-  const int s = A.tile_start[tile], nT = A.tile_start[tile + 1] - s;
-  const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
-  const int nH1 = A.h1count[tile];
-  const int rb = A.rec_start[tile];
-  const int ext0 = A.ext_start[tile];
-  const uint32_t nreg = A.nreg[tile];
-  constexpr int XW = HangExt<D>::WORDS;
-  TileRec trec[KT];
-  uint32_t tsf[KT];  // LDS slot (low 16 bits) and flags (high 16 bits)
+  TileCtx<D> c;
+  c.load(A, tile);
+  const int s = c.s, nT = c.nT, hb = c.hb, nH = c.nH, nH1 = c.nH1;
+  const bool ufast = A.geom.pow2 != 0;
+  RecRegs trec[KT];
+  uint32_t tsf[KT];  // LDS slot of a non-uniform record (low 16 bits) and flags (high 16 bits)
   // the phi_n averaged with in the second RK2 stage: loaded before the first
   // barrier where registers allow (blocks below 1024 threads run under a
   // 128-VGPR cap), else read at the store (under the 64-VGPR cap of the
@@ -302,23 +463,17 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * BLOCK;
-    if (i < nT) {
-      trec[k] = A.rec[rb + i];
-      tsf[k] = uint32_t(A.rec_slot[rb + i]) | (uint32_t(A.flags[s + i]) << 16);
-    }
+    if (i < nT) tsf[k] = uint32_t(fetch_record<D>(A, c, i, trec[k])) | (uint32_t(A.flags[s + i]) << 16);
   }
   // the first K1 rows of H1 records (pass A), also loaded before the barrier
   constexpr int K1 = 2;
-  TileRec hrec[K1];
+  RecRegs hrec[K1];
   int hslot[K1];
   if constexpr (SECOND_ORDER) {
 #pragma unroll
     for (int k = 0; k < K1; ++k) {
       const int i = nT + int(threadIdx.x) + k * BLOCK;
-      if (i < nT + nH1) {
-        hrec[k] = A.rec[rb + i];
-        hslot[k] = A.rec_slot[rb + i];
-      }
+      if (i < nT + nH1) hslot[k] = fetch_record<D>(A, c, i, hrec[k]);
     }
   }
   // the phi gather, unrolled so that a thread's loads are in flight together:
@@ -340,8 +495,10 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   }
   if constexpr (HOLD_PREV) {
 #pragma unroll
-    for (int k = 0; k < KT; ++k)
-      if (int(threadIdx.x) + k * BLOCK < nT) tprev[k] = A.out[s + int(tsf[k] & 0xffffu)];
+    for (int k = 0; k < KT; ++k) {
+      const int i = int(threadIdx.x) + k * BLOCK;
+      if (i < nT) tprev[k] = A.out[s + slot_at<D>(c, i, trec[k], int(tsf[k] & 0xffffu))];
+    }
   }
 #pragma unroll
   for (int k = 0; k < KH; ++k) {
@@ -361,56 +518,51 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   for (int j = int(threadIdx.x) + KH * BLOCK; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();
   if constexpr (SECOND_ORDER) {
-    // rec: the record at tile-local position i, j: its LDS slot
-    auto pass_a = [&](const TileRec &rec, int j, int i) {
-      R f000, fnb[2 * D], dd[D];
-      if (!(rec.geo & GEO_IRREGULAR)) {
-        RegStencil<D, R> st;
-        st.load(rec, A.geom);
-        laplace<D, R>(st, j, phis, f000, fnb, dd);
-      } else {
-        HangStencil<D, R> st;
-        st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
-        laplace<D, R>(st, j, phis, f000, fnb, dd);
-      }
-#pragma unroll
-      for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
-    };
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
       const int i = int(threadIdx.x) + k * BLOCK;
-      if (i < nT) pass_a(trec[k], int(tsf[k] & 0xffffu), i);
+      if (i < nT) tile_pass_a<D, R>(A, c, ufast, trec[k], int(tsf[k] & 0xffffu), i, phis, dds);
     }
 #pragma unroll
     for (int k = 0; k < K1; ++k) {
       const int i = nT + int(threadIdx.x) + k * BLOCK;
-      if (i < nT + nH1) pass_a(hrec[k], hslot[k], i);
+      if (i < nT + nH1) tile_pass_a<D, R>(A, c, ufast, hrec[k], hslot[k], i, phis, dds);
     }
-    for (int i = nT + int(threadIdx.x) + K1 * BLOCK; i < nT + nH1; i += BLOCK)
-      pass_a(A.rec[rb + i], A.rec_slot[rb + i], i);
+    for (int i = nT + int(threadIdx.x) + K1 * BLOCK; i < nT + nH1; i += BLOCK) {
+      RecRegs r;
+      const int js = fetch_record<D>(A, c, i, r);
+      tile_pass_a<D, R>(A, c, ufast, r, js, i, phis, dds);
+    }
     __syncthreads();
   }
 #pragma unroll
   for (int k = 0; k < KT; ++k) {
     const int i = int(threadIdx.x) + k * BLOCK;
     if (i >= nT) continue;
-    const TileRec &rec = trec[k];
-    const int j = int(tsf[k] & 0xffffu);
-    const int64_t n = int64_t(s) + j;
     const uint16_t fl = uint16_t(tsf[k] >> 16);
-    const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
+    int j = int(tsf[k] & 0xffffu);
     R r;
-    if (!(rec.geo & GEO_IRREGULAR)) {
-      RegStencil<D, R> st;
-      st.load(rec, A.geom);
-      r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
-                                                                   phis, dds, DMAX);
+    if (ufast && c.uniform(i)) {
+      UniStencil<D, R> st;
+      j = st.load(trec[k].w, A.geom);
+      r = stage_value<D, R, SECOND_ORDER, UniStencil<D, R>, true>(st, j, fl, 0, A.sfix + (s + i), A.N,
+                                                                   A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
     } else {
-      HangStencil<D, R> st;
-      st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
-      r = stage_value<D, R, SECOND_ORDER, HangStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N, A.phi0 + n,
-                                                                   phis, dds, DMAX);
+      const TileRec rec = record_at<D>(c, i, trec[k], j);
+      const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
+      if (!(rec.geo & GEO_IRREGULAR)) {
+        RegStencil<D, R> st;
+        st.load(rec, A.geom);
+        r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N,
+                                                                     A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
+      } else {
+        HangStencil<D, R> st;
+        st.load(rec, c.ext_of(A.ext, i), A.geom);
+        r = stage_value<D, R, SECOND_ORDER, HangStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N,
+                                                                      A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
+      }
     }
+    const int64_t n = int64_t(s) + j;
     if constexpr (HOLD_PREV)
       A.out[n] = R(0.5) * (tprev[k] + r);
     else if constexpr (RK_AVG)
@@ -422,7 +574,7 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
 
 // The phi_0-only prep of one tile in FP64: phi_0 of T u H1 u H2 into LDS,
 // pass A of phi_0 for T u H1 (second order), then flags and s_fix of the
-// tile's nodes, written in record order (tile start + record rank), the
+// tile's nodes, written in record order (tile start + record position), the
 // order the stage body reads them in.  A.phin is phi_0.
 template <int D, class S, bool SECOND_ORDER, int BLOCK>
 __device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, double *__restrict__ phis,
@@ -430,50 +582,46 @@ __device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, d
                                           double lim_low, double lim_high, double zdt,
                                           uint16_t *__restrict__ flags, S *__restrict__ sfix) {
   constexpr int DMAX = TileLimits<D>::DMAX;
-  const int s = A.tile_start[tile], nT = A.tile_start[tile + 1] - s;
-  const int hb = A.halo_start[tile], nH = A.halo_start[tile + 1] - hb;
-  const int nH1 = A.h1count[tile];
-  const int rb = A.rec_start[tile];
-  const int ext0 = A.ext_start[tile];
-  const uint32_t nreg = A.nreg[tile];
-  constexpr int XW = HangExt<D>::WORDS;
+  TileCtx<D> c;
+  c.load(A, tile);
+  const int s = c.s, nT = c.nT, hb = c.hb, nH = c.nH, nH1 = c.nH1;
+  const bool ufast = A.geom.pow2 != 0;
   for (int j = threadIdx.x; j < nT; j += BLOCK) phis[j] = A.phin[s + j];
   for (int j = threadIdx.x; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();
   if constexpr (SECOND_ORDER) {
     for (int i = int(threadIdx.x); i < nT + nH1; i += BLOCK) {
-      const TileRec rec = A.rec[rb + i];
-      const int j = A.rec_slot[rb + i];
-      double f000, fnb[2 * D], dd[D];
-      if (!(rec.geo & GEO_IRREGULAR)) {
-        RegStencil<D, double> st;
-        st.load(rec, A.geom);
-        laplace<D, double>(st, j, phis, f000, fnb, dd);
-      } else {
-        HangStencil<D, double> st;
-        st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
-        laplace<D, double>(st, j, phis, f000, fnb, dd);
-      }
-#pragma unroll
-      for (int a = 0; a < D; ++a) dds[a * DMAX + j] = dd[a];
+      RecRegs r;
+      const int js = fetch_record<D>(A, c, i, r);
+      tile_pass_a<D, double>(A, c, ufast, r, js, i, phis, dds);
     }
     __syncthreads();
   }
   for (int i = int(threadIdx.x); i < nT; i += BLOCK) {
-    const TileRec rec = A.rec[rb + i];
-    const int j = A.rec_slot[rb + i];
-    const bool masked = mask && mask[s + j] == 0;
+    RecRegs r;
+    int j = fetch_record<D>(A, c, i, r);
     uint16_t fl;
-    if (!(rec.geo & GEO_IRREGULAR)) {
-      RegStencil<D, double> st;
-      st.load(rec, A.geom);
-      fl = prep_value<D, S, RegStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
-                                                                 zdt, SECOND_ORDER, sfix + (s + i), A.N);
+    if (ufast && c.uniform(i)) {
+      UniStencil<D, double> st;
+      j = st.load(r.w, A.geom);
+      const bool masked = mask && mask[s + j] == 0;
+      fl = prep_value<D, S, UniStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
+                                                                  zdt, SECOND_ORDER, sfix + (s + i), A.N);
     } else {
-      HangStencil<D, double> st;
-      st.load(rec, hang_record(A.ext, ext0, nreg, nT, i, XW), A.geom);
-      fl = prep_value<D, S, HangStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low, lim_high,
-                                                                 zdt, SECOND_ORDER, sfix + (s + i), A.N);
+      const TileRec rec = record_at<D>(c, i, r, j);
+      const bool masked = mask && mask[s + j] == 0;
+      if (!(rec.geo & GEO_IRREGULAR)) {
+        RegStencil<D, double> st;
+        st.load(rec, A.geom);
+        fl = prep_value<D, S, RegStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low,
+                                                                   lim_high, zdt, SECOND_ORDER, sfix + (s + i), A.N);
+      } else {
+        HangStencil<D, double> st;
+        st.load(rec, c.ext_of(A.ext, i), A.geom);
+        fl = prep_value<D, S, HangStencil<D, double>, SECOND_ORDER>(st, j, masked, phis, dds, DMAX, lim_low,
+                                                                    lim_high, zdt, SECOND_ORDER, sfix + (s + i),
+                                                                    A.N);
+      }
     }
     flags[s + i] = fl;
   }

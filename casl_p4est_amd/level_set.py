@@ -5,7 +5,8 @@ behaviour): every reinitialize_* method forwards to
 reinitialize_within_range_of_phi_0 (src/my_p4est_level_set.cpp:290-557),
 which here is one call into the native library (casl_reinit).  phi is
 updated in place.  It may be a torch float64 CUDA tensor (kept on the
-device) or a float64 numpy array (copied to the device and back).
+device) or a float64 numpy array (staged through pinned memory to the
+device and back, casl_reinit_host).
 
 Errors: invalid orders raise ValueError (the reference throws
 std::invalid_argument); any other native failure raises CaslError.
@@ -93,6 +94,16 @@ class DeviceStencil:
             out["reason"] = msg.decode() if msg else ""
         return out
 
+    def tile_check(self, tables):
+        """casl_stencil_tile_check: (True, "") when this stencil's tiled
+        encoding equals the host builder's encoding of `tables` (host tables
+        of the same grid) byte for byte, else (False, the first difference)."""
+        eq = ctypes.c_int32()
+        s = tables.struct()
+        L.call("casl_stencil_tile_check", self._h, ctypes.byref(s), ctypes.byref(eq))
+        msg = L.lib().casl_last_error()
+        return bool(eq.value), ("" if eq.value else (msg.decode() if msg else ""))
+
     def _check_field(self, t, name):
         torch = _torch()
         if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.float64:
@@ -126,6 +137,10 @@ class DeviceStencil:
         st = _stream_ptr(torch)
         if graph and mask is not None:
             raise ValueError("graph replay is not available with a mask")
+        if mask is not None and (int(order_space), int(order_time)) != (2, 2):
+            # reinitialize_2nd_order_with_mask is second order in space and
+            # pseudo-time only (src/my_p4est_level_set.h:487-530)
+            raise ValueError("a mask is supported with orders (2, 2) only")
         if mask is None:
             L.call("casl_reinit_graph" if graph else "casl_reinit", self._h, ctypes.c_void_p(phi.data_ptr()),
                    int(order_space), int(order_time), int(n_iter), float(limit_high), float(limit_low),
@@ -135,6 +150,24 @@ class DeviceStencil:
                 raise TypeError("mask must be a uint8 CUDA tensor with one entry per node")
             L.call("casl_reinit_masked", self._h, ctypes.c_void_p(phi.data_ptr()), ctypes.c_void_p(mask.data_ptr()),
                    int(n_iter), float(limit_high), float(limit_low), int(precision), st)
+        return phi
+
+    def reinit_host(self, phi, order_space, order_time, n_iter, limit_high, limit_low, precision, mask=None):
+        """reinitialize_within_range_of_phi_0 on a float64 numpy array in host
+        memory, in place (casl_reinit_host: pinned, chunked staging); mask: a
+        numpy array, nodes with mask == 0 keep their value (orders 2/2)."""
+        torch = _torch()
+        if not isinstance(phi, np.ndarray) or phi.dtype != np.float64 or phi.shape != (self.n_nodes,) \
+                or not phi.flags.c_contiguous or not phi.flags.writeable:
+            raise TypeError("phi must be a writeable contiguous float64 array with one value per node")
+        m = None
+        if mask is not None:
+            m = np.ascontiguousarray(np.asarray(mask) != 0, dtype=np.uint8)
+            if m.shape != (self.n_nodes,):
+                raise TypeError("mask must hold one entry per node")
+        L.call("casl_reinit_host", self._h, ctypes.c_void_p(phi.ctypes.data),
+               ctypes.c_void_p(m.ctypes.data) if m is not None else None, int(order_space), int(order_time),
+               int(n_iter), float(limit_high), float(limit_low), int(precision), _stream_ptr(torch))
         return phi
 
     def _field_out(self, f, out, comps):
@@ -236,15 +269,15 @@ class LevelSet:
         code = self._precision_code(precision)
         dev = self.ngbd.device()
         if isinstance(phi, np.ndarray):
-            torch = _torch()
-            if phi.dtype != np.float64 or phi.shape != (dev.n_nodes,):
-                raise TypeError("phi must be a float64 array with one value per node")
-            t = torch.from_numpy(phi).cuda()
-            m = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.uint8)).cuda() if mask is not None else None
-            dev.reinit(t, order_space, order_pseudotime, number_of_iterations, phi_0_limit_high, phi_0_limit_low,
-                       code, m)
-            phi[:] = t.cpu().numpy()
-            return phi
+            if mask is not None and not isinstance(mask, np.ndarray):
+                mask = mask.cpu().numpy()
+            try:
+                return dev.reinit_host(phi, order_space, order_pseudotime, number_of_iterations, phi_0_limit_high,
+                                       phi_0_limit_low, code, mask)
+            except L.CaslError as e:
+                if e.code == L.CASL_ERR_INVALID_ARGUMENT:
+                    raise ValueError(str(e)) from e
+                raise
         try:
             return dev.reinit(phi, order_space, order_pseudotime, number_of_iterations, phi_0_limit_high,
                               phi_0_limit_low, code, mask, graph=self.graph and mask is None)

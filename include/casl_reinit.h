@@ -119,6 +119,16 @@ int casl_tile_plan(const casl_flat_stencil *tables, int64_t *n_tiles,
                    int64_t *n_records, int64_t *n_irregular,
                    int64_t *n_irregular_nodes, int64_t *n_two_hanging);
 
+/* No counterpart in the reference: a check that the stencil's tiled encoding
+ * (built on the device for casl_device_grid_stencil, on the host for
+ * casl_stencil_create) equals, byte for byte, the host builder's encoding of
+ * the given host tables of the same grid.  *equal = 1 when every array is
+ * the same (or neither could be tiled), else 0 with the first difference in
+ * casl_last_error.  Copies the encoding to the host: a test, not a step of a
+ * solve. */
+int casl_stencil_tile_check(const casl_stencil_t *st, const casl_flat_stencil *tables,
+                            int32_t *equal);
+
 /* second_derivatives_central: fdd[a*N + n] = d^2 f / dx_a^2 at node n.
  * f and fdd are FP64 device arrays. */
 int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd,
@@ -162,6 +172,18 @@ int casl_advect_cfl_dt(casl_stencil_t *st, const double *vn, double *dt,
 int casl_reinit(casl_stencil_t *st, double *phi, int order_space,
                 int order_pseudotime, int n_iter, double limit_high,
                 double limit_low, int precision, void *stream);
+
+/* casl_reinit (mask == NULL) or casl_reinit_masked (orders 2/2 only) with
+ * phi and mask in host memory -- the arrays of the caller's PETSc Vecs
+ * (src/my_p4est_level_set.cpp:318-333) -- phi updated in place; returns when
+ * phi holds the result.  phi travels through a pinned staging buffer kept
+ * with the stencil (allocated on the first call, like the device copy of
+ * phi), in chunks whose host copies overlap the DMA transfers.  Same results
+ * as casl_reinit / casl_reinit_masked on device copies. */
+int casl_reinit_host(casl_stencil_t *st, double *phi, const uint8_t *mask,
+                     int order_space, int order_pseudotime, int n_iter,
+                     double limit_high, double limit_low, int precision,
+                     void *stream);
 
 /* casl_reinit recorded into a hipGraph on its first call and replayed on
  * later calls with the same arguments (phi address included) and the same
@@ -420,7 +442,8 @@ int casl_device_grid_download(const casl_device_grid_t *g, int32_t *nbr,
 int casl_device_grid_sample(const casl_device_grid_t *g,
                             const casl_levelset *phi, double *out,
                             void *stream);
-/* A device stencil handle (as casl_stencil_create) for the grid's tables. */
+/* A device stencil handle (as casl_stencil_create) for the grid's tables,
+ * with its tiled encoding built on the device: the tables never leave it. */
 int casl_device_grid_stencil(const casl_device_grid_t *g,
                              casl_stencil_t **out);
 

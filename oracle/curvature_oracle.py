@@ -300,13 +300,17 @@ def grid_block(tables, n, h):
     return blk
 
 
-def grid_samples(tables, phi, normals, kappa, h, band):
+def grid_samples(tables, phi, normals, kappa, h, band, nodes=None):
     """Rows for every node with |phi| < band that has a uniform block, in node
-    order: (feat [n][3^dim+1] float32, offset [n][dim], node [n], skipped)."""
+    order: (feat [n][3^dim+1] float32, offset [n][dim], node [n], skipped).
+    nodes: only these candidate nodes (in the given order), for grids too
+    large to visit every band node in Python."""
     dim, N = tables.dim, tables.n_nodes
     offs = _block_offsets(dim)
+    cand = np.nonzero(np.abs(phi) < band)[0] if nodes is None else \
+        [int(n) for n in nodes if abs(phi[int(n)]) < band]
     feats, offsets, nodes, skipped = [], [], [], 0
-    for n in np.nonzero(np.abs(phi) < band)[0]:
+    for n in cand:
         blk = grid_block(tables, int(n), h)
         if blk is None:
             skipped += 1

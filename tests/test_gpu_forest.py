@@ -10,6 +10,9 @@ forest (Forest.refine_levelset + build_nodes + construct_neighbors):
     level sets that call sin, cos or atan2;
   * reinitialize_2nd_order on the device-built stencil equals the host-built
     one bit for bit;
+  * the tiled encoding of the device-built stencil, made on the device from
+    tables that never leave it, equals the host builder's encoding of the
+    host tables byte for byte (or neither exists: non-cubic cells);
   * the level sets refine the measured workloads (star2d, spheres3d at full
     size, gyroid3d at lmax 9) to the same grids.
 """
@@ -78,6 +81,29 @@ def test_device_grid_equals_host(cuda, case):
     C.DeviceStencil(tab).reinit(a, 2, 2, 5, C.DBL_MAX, -C.DBL_MAX, 0)
     g.device().reinit(b, 2, 2, 5, C.DBL_MAX, -C.DBL_MAX, 0)
     assert torch.equal(a, b)
+    ok, why = g.device().tile_check(tab)
+    assert ok, why
+
+
+@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5])], ids=["rect2d", "brick3d"])
+def test_device_tiling_noncubic_cells(cuda, dim, box_max):
+    """Cells that are not cubes: neither the host nor the device builds the
+    tiled encoding; the device-built stencil runs the two-pass path with the
+    host-built one's results."""
+    import torch
+    phi = ls.Sphere([0.9] + [0.45] * (dim - 1), 0.3) if dim == 2 else ls.Sphere([0.9, 0.5, 0.25], 0.2)
+    f, tab = host_grid(dim, phi, 2, 7 if dim == 2 else 6, 2.4, 0.0, [0.0] * dim, box_max)
+    g = C.DeviceGrid(dim, phi, 2, 7 if dim == 2 else 6, lip=2.4, box_min=[0.0] * dim, box_max=box_max)
+    ds = g.device()
+    assert not ds.tile_info()["tiled"]
+    assert ds.path == "two_pass"
+    ok, why = ds.tile_check(tab)
+    assert ok, why
+    a = torch.from_numpy(f.sample(phi)).to(cuda)
+    b = a.clone()
+    C.DeviceStencil(tab).reinit(a, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0)
+    ds.reinit(b, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0)
+    assert torch.equal(a, b)
 
 
 def test_device_grid_root_only_fails_like_host(cuda):
@@ -140,3 +166,11 @@ def test_device_grid_workloads(cuda, name, lmax):
     t2 = time.time()
     say(f"    {f.num_nodes} nodes: host {t1 - t0:.2f} s, device {t2 - t1:.2f} s")
     assert_same_grid(f, tab, g)
+    t3 = time.time()
+    ds = g.device()
+    t4 = time.time()
+    info = ds.tile_info()
+    say(f"    device tiling {t4 - t3:.3f} s: {info}")
+    assert info["tiled"]
+    ok, why = ds.tile_check(tab)
+    assert ok, why

@@ -244,10 +244,25 @@ def run_curvature(args, torch):
         CV.hybrid_hk(net, f)
     torch.cuda.synchronize()
     t_ml = (time.perf_counter() - a) / reps
+    a = time.perf_counter()
+    for _ in range(reps):
+        CV.hybrid_hk(net, f, fused=False)
+    torch.cuda.synchronize()
+    t_sep = (time.perf_counter() - a) / reps
     n_s = f.shape[0]
+    # the fused kernel alone (HIP events): BF16 MFMA rate over its 2 network rows per sample
+    lib.casl_profile_enable(1)
+    lib.casl_profile_reset()
+    for _ in range(reps):
+        CV.hybrid_hk(net, f)
+    L.call("casl_profile_query", b"k_net_curvature", ctypes.byref(ms), ctypes.byref(cnt))
+    lib.casl_profile_enable(0)
+    fused_ms = ms.value / max(1, cnt.value)
+    fused_tflops = 2 * 2 * macs * n_s / (fused_ms * 1e-3) / 1e12
     log(f"[bench] curvature: mean |hk err| numerical {e_num:.3e}, learned {e_ml:.3e} (x{e_num / e_ml:.2f}); "
         f"training {hist['seconds']:.1f}s, GEMM {gemm_tflops:.1f} TFLOP/s; per sample numerical "
-        f"{1e9 * t_num / n_s:.1f} ns, network {1e9 * t_ml / n_s:.1f} ns")
+        f"{1e9 * t_num / n_s:.1f} ns, network + blend fused {1e9 * t_ml / n_s:.2f} ns (kernel {fused_ms:.3f} ms, "
+        f"{fused_tflops:.1f} TFLOP/s), as three calls {1e9 * t_sep / n_s:.1f} ns")
     return {
         "metric": "learned h*kappa mean |error| on held-out circles and sinusoids (lower is better)",
         "value": e_ml,
@@ -275,7 +290,11 @@ def run_curvature(args, torch):
         "roofline": {"bound": "mfma", "kernel": "k_gemm", "achieved": gemm_tflops, "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": gemm_tflops / 2500.0, "traffic": None,
                      "note": "training GEMMs are 128 wide: far from the dense BF16 peak"},
-        "gpu_ns_per_sample": {"numerical": 1e9 * t_num / n_s, "network": 1e9 * t_ml / n_s},
+        "gpu_ns_per_sample": {"numerical": 1e9 * t_num / n_s, "network": 1e9 * t_ml / n_s,
+                              "network_as_three_calls": 1e9 * t_sep / n_s, "samples": int(n_s)},
+        "fused_network_kernel": {"kernel": "k_net_curvature", "avg_ms": fused_ms, "achieved_tflops": fused_tflops,
+                                 "peak_tflops": 2500.0, "frac": fused_tflops / 2500.0,
+                                 "flops_per_sample": 2 * 2 * macs},
         "cpu_baseline": None,
     }
 
@@ -295,8 +314,20 @@ def run_curvature3d(args, torch):
         f"{time.time() - t0:.1f}s (reinit {tr['reinit_s']:.1f}s, sampling {tr['samples_s']:.2f}s)")
     net = CV.CurvatureNet(CV.DIMS_3D, seed=SEED % 1000)
     hist = CV.train(net, tr["feat"], tr["target"], epochs=args.epochs, batch=16384, lr=2e-3, log=log)
+    # the hybrid's deadband: the smallest mean error on validation shapes
+    # (their own seed; neither the test shapes below nor the gyroid)
+    va = CV.make_dataset_3d(box, 100, 100, seed=SEED + 3, per_shape=1024)
+    vt = va["target"]
+    dead_err = {}
+    for dead in (0.0, 1.25e-4, 2.5e-4, 5e-4, 1e-3, 2e-3, 4e-3):
+        hv = CV.hybrid_hk(net, va["feat"], deadband=dead).cpu().numpy().astype(np.float64)
+        dead_err[dead] = float(np.abs(hv - vt).mean())
+    net.deadband = min(dead_err, key=lambda d: (dead_err[d], d))
+    log(f"[bench] curvature3d deadband {net.deadband:g} from validation errors "
+        + ", ".join(f"{d:g}: {e:.4e}" for d, e in dead_err.items()))
     if args.save_weights:
-        net.save(args.save_weights, note=f"bench.py --config curvature3d --epochs {args.epochs} (seed {SEED})")
+        net.save(args.save_weights, note=f"bench.py --config curvature3d --epochs {args.epochs} (seed {SEED}); "
+                                         f"deadband {net.deadband:g} chosen on validation shapes (seed {SEED + 3})")
         log(f"[bench] weights saved to {args.save_weights}")
     feat, t = te["feat"], te["target"]
     hk = CV.hybrid_hk(net, feat).cpu().numpy().astype(np.float64)
@@ -338,6 +369,8 @@ def run_curvature3d(args, torch):
         "learned_over_numerical": e_ml / e_num,
         "by_shape": kinds,
         "by_numerical_hk": bins,
+        "deadband": {"chosen": net.deadband, "validation_error": {f"{d:g}": e for d, e in dead_err.items()},
+                     "validation_samples": int(len(vt))},
         "error_where_network_used": {"numerical": float(np.abs(ihk[big] - t[big]).mean()),
                                      "learned": float(np.abs(hk[big] - t[big]).mean()), "samples": int(big.sum())},
         "training": {"seconds": hist["seconds"], "rows_per_s": hist["rows"] / hist["seconds"],

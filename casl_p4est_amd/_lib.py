@@ -136,6 +136,9 @@ SIGNATURES = {
     "casl_curvature_hybrid_3d": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_float,
                                                 ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
                                                 ctypes.c_void_p]),
+    "casl_curvature_learned": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_float,
+                                              ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_void_p,
+                                              ctypes.c_void_p]),
     "casl_mlp_create": (ctypes.c_int, [c_int32_p, ctypes.c_int, ctypes.c_uint64, ctypes.POINTER(ctypes.c_void_p)]),
     "casl_mlp_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "casl_mlp_num_params": (ctypes.c_int, [ctypes.c_void_p, c_int64_p, c_int64_p]),

@@ -12,9 +12,11 @@
 // reference's "ihk", src/my_p4est_curvature_ml.cpp:414-480).
 #include <hip/hip_runtime.h>
 
+#include "curv_device.hpp"
 #include "reinit_device.hpp"
 
 using namespace casl;
+using namespace casl::dev;
 
 namespace {
 
@@ -118,39 +120,7 @@ __global__ void k_zero_count(unsigned long long *c) { *c = 0ull; }
 __global__ void k_normalize(const float *__restrict__ feat, int64_t n, float *__restrict__ out) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  float f[10], g[9];
-#pragma unroll
-  for (int q = 0; q < 10; ++q) f[q] = feat[10 * s + q];
-  const float sg = f[9] > 0.f ? -1.f : 1.f;
-#pragma unroll
-  for (int q = 0; q < 10; ++q) f[q] *= sg;
-  float gx = f[7] - f[1], gy = f[5] - f[3];
-  if (gx == 0.f) gx = float(dev::EPS);
-  if (gy == 0.f) gy = float(dev::EPS);
-  // index maps of the rotated stencil: new[q] = old[map[q]]
-  constexpr int CCW[9] = {2, 5, 8, 1, 4, 7, 0, 3, 6};  // +90 degrees
-  constexpr int CW[9] = {6, 3, 0, 7, 4, 1, 8, 5, 2};   // -90 degrees
-  constexpr int HALF[9] = {8, 7, 6, 5, 4, 3, 2, 1, 0};  // 180 degrees
-  const int quad = gy > 0.f ? (gx > 0.f ? 1 : 2) : (gx < 0.f ? 3 : 4);
-#pragma unroll
-  for (int q = 0; q < 9; ++q) g[q] = f[q];
-  if (quad == 2) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) f[q] = g[CW[q]];
-  } else if (quad == 3) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) f[q] = g[HALF[q]];
-  } else if (quad == 4) {
-#pragma unroll
-    for (int q = 0; q < 9; ++q) f[q] = g[CCW[q]];
-  }
-  float *o = out + 20 * s;
-#pragma unroll
-  for (int q = 0; q < 10; ++q) o[q] = f[q];
-  constexpr int REFL[9] = {0, 3, 6, 1, 4, 7, 2, 5, 8};  // swap 1-3, 2-6, 5-7
-#pragma unroll
-  for (int q = 0; q < 9; ++q) o[10 + q] = f[REFL[q]];
-  o[19] = f[9];
+  curv_normalize2(feat + 10 * s, out + 20 * s);
 }
 
 // Hybrid h*kappa (src/my_p4est_curvature_ml.cpp:483-556): the numerical value
@@ -161,18 +131,7 @@ __global__ void k_hybrid(const float *__restrict__ feat, const float *__restrict
                          float hi, int residual, float *__restrict__ hk) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const float ihk = feat[10 * s + 9], a = fabsf(ihk);
-  if (a < lo) {
-    hk[s] = ihk;
-    return;
-  }
-  float p = 0.5f * (pred[2 * s] + pred[2 * s + 1]);
-  if (residual) p += -a;
-  if (a <= hi) {
-    const float lam = (hi - a) / (hi - lo);
-    p = (1.f - lam) * p + lam * -a;
-  }
-  hk[s] = ihk > 0.f ? fabsf(p) : -fabsf(p);
+  hk[s] = curv_hybrid2(feat[10 * s + 9], pred[2 * s], pred[2 * s + 1], lo, hi, residual);
 }
 
 }  // namespace

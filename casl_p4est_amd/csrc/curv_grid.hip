@@ -30,6 +30,7 @@
 //             corrections at most `deadband` in size are dropped.
 #include <hip/hip_runtime.h>
 
+#include "curv_device.hpp"
 #include "reinit_device.hpp"
 
 using namespace casl;
@@ -150,72 +151,9 @@ __global__ void k_zero2(unsigned long long *c) {
 // [n][28] samples -> [n][28] network inputs (one canonical orientation)
 __global__ void __launch_bounds__(BLOCK) k_normalize3(const float *__restrict__ feat, int64_t n,
                                                       float *__restrict__ out) {
-  using B = Block<3>;
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  float f[28];
-#pragma unroll
-  for (int q = 0; q < 28; ++q) f[q] = feat[28 * s + q];
-  const float sg = f[27] > 0.f ? -1.f : 1.f;
-#pragma unroll
-  for (int q = 0; q < 28; ++q) f[q] *= sg;
-  float g[3] = {f[B::q(1, 0, 0)] - f[B::q(-1, 0, 0)], f[B::q(0, 1, 0)] - f[B::q(0, -1, 0)],
-                f[B::q(0, 0, 1)] - f[B::q(0, 0, -1)]};
-  int r[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (g[a] == 0.f) g[a] = float(EPS);
-    r[a] = g[a] < 0.f ? -1 : 1;
-    g[a] = fabsf(g[a]);
-  }
-  // axes by decreasing |difference|, ties to the lower axis
-  int p[3] = {0, 1, 2};
-  if (g[p[1]] > g[p[0]]) {
-    const int x = p[0];
-    p[0] = p[1];
-    p[1] = x;
-  }
-  if (g[p[2]] > g[p[1]]) {
-    const int x = p[1];
-    p[1] = p[2];
-    p[2] = x;
-    if (g[p[1]] > g[p[0]]) {
-      const int y = p[0];
-      p[0] = p[1];
-      p[1] = y;
-    }
-  }
-  float v[27];
-  for (int i = -1; i <= 1; ++i)
-    for (int j = -1; j <= 1; ++j)
-      for (int k = -1; k <= 1; ++k) {
-        // new axis b reads old axis p[b]
-        int c[3];
-        c[p[0]] = i * r[p[0]];
-        c[p[1]] = j * r[p[1]];
-        c[p[2]] = k * r[p[2]];
-        v[B::q(i, j, k)] = f[B::q(c[0], c[1], c[2])];
-      }
-  // the same 27 values in a basis where curvature is not buried under the
-  // linear part (BF16 keeps 8 bits of each input): the centre value, the
-  // three half centred differences g at the -e_a slots, and everywhere else
-  // the deviation from the plane f0 + o.g
-  const float f0 = v[B::q(0, 0, 0)];
-  const float gx = 0.5f * (v[B::q(1, 0, 0)] - v[B::q(-1, 0, 0)]);
-  const float gy = 0.5f * (v[B::q(0, 1, 0)] - v[B::q(0, -1, 0)]);
-  const float gz = 0.5f * (v[B::q(0, 0, 1)] - v[B::q(0, 0, -1)]);
-  float *o = out + 28 * s;
-  for (int i = -1; i <= 1; ++i)
-    for (int j = -1; j <= 1; ++j)
-      for (int k = -1; k <= 1; ++k) {
-        const int q = B::q(i, j, k);
-        o[q] = v[q] - (f0 + (float(i) * gx + float(j) * gy + float(k) * gz));
-      }
-  o[B::q(0, 0, 0)] = f0;
-  o[B::q(-1, 0, 0)] = gx;
-  o[B::q(0, -1, 0)] = gy;
-  o[B::q(0, 0, -1)] = gz;
-  o[27] = f[27];
+  curv_normalize3(feat + 28 * s, out + 28 * s);
 }
 
 // Hybrid h*kappa of 3D samples: one network output per sample.  With a
@@ -227,18 +165,7 @@ __global__ void __launch_bounds__(BLOCK) k_hybrid3(const float *__restrict__ fea
                                                    float *__restrict__ hk) {
   const int64_t s = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= n) return;
-  const float ihk = feat[28 * s + 27], a = fabsf(ihk);
-  if (a < lo) {
-    hk[s] = ihk;
-    return;
-  }
-  float p = pred[s];
-  if (residual) p = (fabsf(p) > dead ? p : 0.f) + -a;
-  if (a <= hi) {
-    const float lam = (hi - a) / (hi - lo);
-    p = (1.f - lam) * p + lam * -a;
-  }
-  hk[s] = ihk > 0.f ? fabsf(p) : -fabsf(p);
+  hk[s] = curv_hybrid3(feat[28 * s + 27], pred[s], lo, hi, residual, dead);
 }
 
 }  // namespace

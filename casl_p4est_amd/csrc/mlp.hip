@@ -23,9 +23,14 @@
 #include <random>
 #include <vector>
 
+#include "curv_device.hpp"
 #include "reinit_device.hpp"
 
 using namespace casl;
+using casl::dev::curv_hybrid2;
+using casl::dev::curv_hybrid3;
+using casl::dev::curv_normalize2;
+using casl::dev::curv_normalize3;
 
 namespace {
 
@@ -267,6 +272,151 @@ void forward(casl_mlp *m, const float *x, int n, float *out, hipStream_t s) {
   }
 }
 
+// ------------------------------------------------ fused network curvature
+// The learned h*kappa of samples in one launch: input normalisation, every
+// layer of a (10|28)-128-128-128-128-1 network on MFMA with the activations
+// in LDS, and the hybrid blend.  Each block holds all weights in LDS (stored
+// [out][in], so a column's 8 consecutive k are one 16-byte read) and runs
+// tiles of 128 network rows: 2D samples give 2 rows each (as is, reflected),
+// 3D samples one.  Wave w owns rows 32w .. 32w + 31 in every layer: it reads
+// only its own rows of X and overwrites them with the layer's output, so the
+// layers need no barrier.  Every product, bias add, ReLU and BF16 rounding is
+// the one k_gemm makes, in the same order (the K loop ascends in steps of
+// 16), so the result equals casl_curvature_normalize(_3d) + casl_mlp_forward
+// + casl_curvature_hybrid(_3d) bit for bit.
+constexpr int NF_H = 128;            // hidden width
+constexpr int NF_XS = NF_H + 8;      // LDS row stride (halves) of X and of the hidden weights
+constexpr int NF_K0S = 32 + 8;       // LDS row stride of the first layer's weights
+constexpr int NF_ROWS = 128;         // network rows per tile
+
+struct FusedNet {
+  const uint16_t *W16;  // [in][out] per layer, concatenated
+  const float *b;       // per layer, concatenated
+  const float *mean, *inv_std;
+  size_t offW[5], offb[5];
+  int in;               // 10 or 28
+};
+
+// LDS footprint (bytes): weights, X, biases, predictions
+constexpr size_t NF_LDS = size_t(NF_H) * NF_K0S * 2 + 3 * size_t(NF_H) * NF_XS * 2 + size_t(32) * NF_XS * 2 +
+                          size_t(NF_ROWS) * NF_XS * 2 + (4 * NF_H + 1) * 4 + NF_ROWS * 4;
+static_assert(NF_LDS <= 160 * 1024, "fused network kernel exceeds the LDS of a compute unit");
+
+template <int D>
+__global__ void __launch_bounds__(256)
+    k_net_curvature(FusedNet net, const float *__restrict__ feat, int64_t n, float lo, float hi, int residual,
+                    float dead, float *__restrict__ hk) {
+  constexpr int NFEAT = D == 2 ? 10 : 28;                 // features per sample
+  constexpr int RPS = D == 2 ? 2 : 1;                     // network rows per sample
+  constexpr int SPT = NF_ROWS / RPS;                      // samples per tile
+  extern __shared__ __align__(16) unsigned char lds[];
+  uint16_t *W0 = reinterpret_cast<uint16_t *>(lds);                 // [128][NF_K0S]
+  uint16_t *Wh = W0 + NF_H * NF_K0S;                                // 3 x [128][NF_XS]
+  uint16_t *W4 = Wh + 3 * NF_H * NF_XS;                             // [32][NF_XS], row 0 = last layer
+  uint16_t *X = W4 + 32 * NF_XS;                                    // [128][NF_XS]
+  float *bias = reinterpret_cast<float *>(X + NF_ROWS * NF_XS);     // 4 x 128 + 1
+  float *pred = bias + 4 * NF_H + 1;                                // [128]
+  const int tid = int(threadIdx.x), wave = tid >> 6, lane = tid & 63;
+  const int in = net.in, k0p = (in + 15) / 16 * 16;
+  // weights, transposed into [out][in], zero beyond each layer's width
+  for (int e = tid; e < NF_H * 32; e += 256) {
+    const int j = e / 32, k = e % 32;
+    W0[j * NF_K0S + k] = k < in ? net.W16[net.offW[0] + size_t(k) * NF_H + j] : uint16_t(0);
+  }
+  for (int l = 1; l <= 3; ++l)
+    for (int e = tid; e < NF_H * NF_H; e += 256) {
+      const int k = e / NF_H, j = e % NF_H;  // coalesced over j in global memory
+      Wh[(l - 1) * NF_H * NF_XS + j * NF_XS + k] = net.W16[net.offW[l] + size_t(k) * NF_H + j];
+    }
+  for (int e = tid; e < 32 * NF_H; e += 256) {
+    const int j = e / NF_H, k = e % NF_H;
+    W4[j * NF_XS + k] = j == 0 ? net.W16[net.offW[4] + size_t(k)] : uint16_t(0);
+  }
+  for (int e = tid; e < 4 * NF_H + 1; e += 256) {
+    const int l = e / NF_H, j = e % NF_H;
+    bias[e] = net.b[net.offb[l] + size_t(j)];
+  }
+  const int64_t ntiles = (n + SPT - 1) / SPT;
+  const int li = lane & 31, lk = (lane >> 5) * 8;
+  const int row0 = wave * 32;
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    __syncthreads();  // weights loaded; the previous tile's X and pred consumed
+    // inputs: normalised, standardised, BF16 (k_pack_input's rounding)
+    for (int t = tid; t < SPT; t += 256) {
+      const int64_t smp = tile * SPT + t;
+      float o[RPS * NFEAT];
+      if (smp < n) {
+        if constexpr (D == 2)
+          curv_normalize2(feat + NFEAT * smp, o);
+        else
+          curv_normalize3(feat + NFEAT * smp, o);
+      }
+#pragma unroll
+      for (int r = 0; r < RPS; ++r) {
+        uint16_t *xr = X + (t * RPS + r) * NF_XS;
+        for (int k = 0; k < 32; ++k)
+          xr[k] = (smp < n && k < in) ? f2bf((o[r * NFEAT + k] - net.mean[k]) * net.inv_std[k]) : uint16_t(0);
+      }
+    }
+    __syncthreads();
+    // hidden layers: wave-owned rows, in place
+    for (int l = 0; l < 4; ++l) {
+      const uint16_t *W = l == 0 ? W0 : Wh + (l - 1) * NF_H * NF_XS;
+      const int ws = l == 0 ? NF_K0S : NF_XS;
+      const int K = l == 0 ? k0p : NF_H;
+      f32x16 acc[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[c][r] = 0.f;
+      for (int k0 = 0; k0 < K; k0 += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(X + (row0 + li) * NF_XS + k0 + lk);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          const bf16x8 b = *reinterpret_cast<const bf16x8 *>(W + (32 * c + li) * ws + k0 + lk);
+          acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc[c], 0, 0, 0);
+        }
+      }
+      const float *bl = bias + l * NF_H;
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = row0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3), col = 32 * c + li;
+          float v = acc[c][r];
+          v += bl[col];
+          v = v > 0.f ? v : 0.f;
+          X[row * NF_XS + col] = f2bf(v);
+        }
+    }
+    // last layer (one output column)
+    {
+      f32x16 acc;
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+      for (int k0 = 0; k0 < NF_H; k0 += 16) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8 *>(X + (row0 + li) * NF_XS + k0 + lk);
+        const bf16x8 b = *reinterpret_cast<const bf16x8 *>(W4 + li * NF_XS + k0 + lk);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      }
+      if (li == 0) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) pred[row0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3)] = acc[r] + bias[4 * NF_H];
+      }
+    }
+    __syncthreads();
+    for (int t = tid; t < SPT; t += 256) {
+      const int64_t smp = tile * SPT + t;
+      if (smp >= n) continue;
+      const float ihk = feat[NFEAT * smp + NFEAT - 1];
+      if constexpr (D == 2)
+        hk[smp] = curv_hybrid2(ihk, pred[2 * t], pred[2 * t + 1], lo, hi, residual);
+      else
+        hk[smp] = curv_hybrid3(ihk, pred[t], lo, hi, residual, dead);
+    }
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -320,6 +470,49 @@ int casl_mlp_create(const int32_t *dims, int n_layers, uint64_t seed, casl_mlp_t
     HIP_CHECK(hipMemcpy(m->inv_std.p, inv.data(), inv.size() * 4, hipMemcpyHostToDevice));
     m->loss.ensure(1);
     *out = m.release();
+  });
+}
+
+int casl_curvature_learned(casl_mlp_t *m, const float *feat, int64_t n, float lo, float hi, int residual,
+                           float deadband, float *hk, void *stream) {
+  return guarded([&]() {
+    if (!m || n < 0 || (n > 0 && (!feat || !hk)) || !(hi > lo) || !(lo >= 0.f) || !(deadband >= 0.f))
+      throw std::invalid_argument("bad arguments");
+    if (m->L != 5 || (m->dim[0] != 10 && m->dim[0] != 28) || m->dim[1] != NF_H || m->dim[2] != NF_H ||
+        m->dim[3] != NF_H || m->dim[4] != NF_H || m->dim[5] != 1)
+      throw std::invalid_argument("casl_curvature_learned takes a (10|28)-128-128-128-128-1 network");
+    if (n == 0) return;
+    FusedNet net{};
+    net.W16 = m->W16.p;
+    net.b = m->b.p;
+    net.mean = m->mean.p;
+    net.inv_std = m->inv_std.p;
+    for (int l = 0; l < 5; ++l) {
+      net.offW[l] = m->offW[l];
+      net.offb[l] = m->offb[l];
+    }
+    net.in = m->dim[0];
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    int dev = 0, cus = 0;
+    HIP_CHECK(hipGetDevice(&dev));
+    HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int64_t spt = net.in == 10 ? NF_ROWS / 2 : NF_ROWS;
+    const int64_t ntiles = (n + spt - 1) / spt;
+    const unsigned grid = unsigned(std::max<int64_t>(1, std::min<int64_t>(ntiles, int64_t(cus))));
+    prof_begin("k_net_curvature", s);
+    if (net.in == 10) {
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_net_curvature<2>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(NF_LDS)));
+      hipLaunchKernelGGL(k_net_curvature<2>, dim3(grid), dim3(256), NF_LDS, s, net, feat, n, lo, hi, residual,
+                         deadband, hk);
+    } else {
+      HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&k_net_curvature<3>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, int(NF_LDS)));
+      hipLaunchKernelGGL(k_net_curvature<3>, dim3(grid), dim3(256), NF_LDS, s, net, feat, n, lo, hi, residual,
+                         deadband, hk);
+    }
+    prof_end("k_net_curvature", s);
+    HIP_CHECK(hipGetLastError());
   });
 }
 

@@ -517,6 +517,16 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   }
   for (int j = int(threadIdx.x) + KH * BLOCK; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();
+  // the phi_n averaged with: loaded here, its latency hidden by pass A
+  constexpr bool LATE_PREV = RK_AVG && !HOLD_PREV;
+  R lprev[LATE_PREV ? KT : 1];
+  if constexpr (LATE_PREV) {
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      const int i = int(threadIdx.x) + k * BLOCK;
+      if (i < nT) lprev[k] = A.out[s + slot_at<D>(c, i, trec[k], int(tsf[k] & 0xffffu))];
+    }
+  }
   if constexpr (SECOND_ORDER) {
 #pragma unroll
     for (int k = 0; k < KT; ++k) {
@@ -535,40 +545,53 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
     }
     __syncthreads();
   }
-#pragma unroll
-  for (int k = 0; k < KT; ++k) {
-    const int i = int(threadIdx.x) + k * BLOCK;
-    if (i >= nT) continue;
-    const uint16_t fl = uint16_t(tsf[k] >> 16);
-    int j = int(tsf[k] & 0xffffu);
-    R r;
-    if (ufast && c.uniform(i)) {
-      UniStencil<D, R> st;
-      j = st.load(trec[k].w, A.geom);
-      r = stage_value<D, R, SECOND_ORDER, UniStencil<D, R>, true>(st, j, fl, 0, A.sfix + (s + i), A.N,
-                                                                   A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
-    } else {
-      const TileRec rec = record_at<D>(c, i, trec[k], j);
-      const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
-      if (!(rec.geo & GEO_IRREGULAR)) {
-        RegStencil<D, R> st;
-        st.load(rec, A.geom);
-        r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N,
-                                                                     A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
-      } else {
-        HangStencil<D, R> st;
-        st.load(rec, c.ext_of(A.ext, i), A.geom);
-        r = stage_value<D, R, SECOND_ORDER, HangStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N,
-                                                                      A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
-      }
-    }
-    const int64_t n = int64_t(s) + j;
+  // pass B, the result of a tile node, stored (or averaged into out)
+  auto store = [&](int k, int64_t n, R r) {
     if constexpr (HOLD_PREV)
       A.out[n] = R(0.5) * (tprev[k] + r);
+    else if constexpr (LATE_PREV)
+      A.out[n] = R(0.5) * (lprev[k] + r);
     else if constexpr (RK_AVG)
       A.out[n] = R(0.5) * (A.out[n] + r);
     else
       A.out[n] = r;
+  };
+  // Pass B in two loops: uniform records, then the others.  As branches of
+  // one loop body the compiler sinks their common Godunov tail into shared
+  // code with per-face registers, and the uniform records would run the
+  // general tail.
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int i = int(threadIdx.x) + k * BLOCK;
+    if (!(ufast && i < c.nTu)) continue;
+    const uint16_t fl = uint16_t(tsf[k] >> 16);
+    UniStencil<D, R> st;
+    const int j = st.load(trec[k].w, A.geom);
+    const R r = stage_value<D, R, SECOND_ORDER, UniStencil<D, R>, true>(st, j, fl, 0, A.sfix + (s + i), A.N,
+                                                                          A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
+    store(k, int64_t(s) + j, r);
+  }
+#pragma unroll
+  for (int k = 0; k < KT; ++k) {
+    const int i = int(threadIdx.x) + k * BLOCK;
+    if (i >= nT || (ufast && i < c.nTu)) continue;
+    const uint16_t fl = uint16_t(tsf[k] >> 16);
+    int j = int(tsf[k] & 0xffffu);
+    R r;
+    const TileRec rec = record_at<D>(c, i, trec[k], j);
+    const uint8_t w = uint8_t((rec.geo >> 24) & 63u);
+    if (!(rec.geo & GEO_IRREGULAR)) {
+      RegStencil<D, R> st;
+      st.load(rec, A.geom);
+      r = stage_value<D, R, SECOND_ORDER, RegStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N,
+                                                                   A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
+    } else {
+      HangStencil<D, R> st;
+      st.load(rec, c.ext_of(A.ext, i), A.geom);
+      r = stage_value<D, R, SECOND_ORDER, HangStencil<D, R>, true>(st, j, fl, w, A.sfix + (s + i), A.N,
+                                                                    A.phi0 + (int64_t(s) + j), phis, dds, DMAX);
+    }
+    store(k, int64_t(s) + j, r);
   }
 }
 

@@ -223,6 +223,9 @@ class CurvatureNet:
         self.n_weights, self.n_biases = nw.value, nb.value
         self.mean = np.zeros(self.dims[0], np.float32)
         self.std = np.ones(self.dims[0], np.float32)
+        # 3D hybrid: predicted corrections up to this size are dropped (chosen
+        # after training on validation shapes, stored with the weights)
+        self.deadband = DEADBAND_3D
         if zero_last_layer:
             # a correction network then starts at zero: the numerical value
             w, b = self.params()
@@ -261,7 +264,7 @@ class CurvatureNet:
         """Weights, biases, input scaler and widths as an .npz (no pickles)."""
         w, b = self.params()
         np.savez(path, dims=np.asarray(self.dims, np.int32), w=w, b=b, mean=self.mean, std=self.std,
-                 note=np.asarray(note))
+                 deadband=np.float32(self.deadband), note=np.asarray(note))
 
     @classmethod
     def load(cls, path):
@@ -269,6 +272,8 @@ class CurvatureNet:
         net = cls(tuple(int(d) for d in z["dims"]), zero_last_layer=False)
         net.set_params(z["w"], z["b"])
         net.set_scaler(z["mean"], z["std"])
+        if "deadband" in z.files:
+            net.deadband = float(z["deadband"])
         return net
 
     def grads(self):
@@ -380,16 +385,30 @@ def train(net, feat, target, epochs=10, batch=16384, lr=1e-3, lr_final=1e-5, see
     return dict(loss=hist, seconds=seconds, steps=step, rows=step * batch, target_scale=sigma)
 
 
-def hybrid_hk(net, feat, residual=True, chunk=1 << 22):
-    """Learned h*kappa of samples feat ([n][10] or [n][28], device), in
-    chunks of at most `chunk` samples (the network's activations are sized by
-    the rows of one call)."""
+def hybrid_hk(net, feat, residual=True, chunk=1 << 22, fused=True, deadband=None):
+    """Learned h*kappa of samples feat ([n][10] or [n][28], device).
+
+    fused (the default for the (10|28)-128x4-1 networks): one
+    casl_curvature_learned launch that normalises, evaluates the network and
+    blends, with the weights and activations in LDS.  Otherwise the three
+    calls (normalisation, casl_mlp_forward, hybrid) in chunks of at most
+    `chunk` samples (the network's activations are sized by the rows of one
+    call).  Both give the same bits.  deadband (3D): the hybrid's deadband,
+    the network's own (chosen after training) when None."""
     torch = _torch()
+    dead = float(getattr(net, "deadband", DEADBAND_3D)) if deadband is None else float(deadband)
+    if fused and net.dims in (DIMS_2D, DIMS_3D) and feat.shape[1] == net.dims[0]:
+        if not feat.is_contiguous() or feat.dtype != torch.float32 or feat.device.type != "cuda":
+            raise TypeError("feat must be a contiguous float32 CUDA tensor")
+        hk = torch.empty(feat.shape[0], dtype=torch.float32, device=feat.device)
+        L.call("casl_curvature_learned", net._h, _ptr(feat), int(feat.shape[0]), float(LO_MIN_HK),
+               float(UP_MIN_HK), int(bool(residual)), dead, _ptr(hk), _stream_ptr(torch))
+        return hk
     if feat.shape[0] > chunk:
-        return torch.cat([hybrid_hk(net, feat[i:i + chunk], residual, chunk)
+        return torch.cat([hybrid_hk(net, feat[i:i + chunk], residual, chunk, False, deadband)
                           for i in range(0, feat.shape[0], chunk)])
     if feat.shape[1] == 28:
-        return hybrid_3d(feat, net.forward(normalize_3d(feat)), residual=residual)
+        return hybrid_3d(feat, net.forward(normalize_3d(feat)), residual=residual, deadband=dead)
     x = normalize(feat)
     y = net.forward(x)
     return hybrid(feat, y, residual=residual)

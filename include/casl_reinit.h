@@ -324,6 +324,17 @@ int casl_curvature_hybrid_3d(const float *feat, const float *pred, int64_t n,
                              float *hk, void *stream);
 
 typedef struct casl_mlp casl_mlp_t;
+/* The learned h*kappa of n samples in one launch, for a network of widths
+ * (10|28)-128-128-128-128-1 (2D | 3D inputs): casl_curvature_normalize
+ * (_3d), casl_mlp_forward and casl_curvature_hybrid(_3d) fused, the weights
+ * and activations held in LDS, every layer on BF16 MFMA with FP32
+ * accumulation.  The same values, bit for bit, as those calls in a row
+ * (deadband applies to 3D only), without their intermediate arrays.  Replaces
+ * kml::Curvature::_computeHybridHK's network evaluation and blend
+ * (src/my_p4est_curvature_ml.cpp:483-556, the network itself :214-286). */
+int casl_curvature_learned(casl_mlp_t *m, const float *feat, int64_t n,
+                           float lo, float hi, int residual, float deadband,
+                           float *hk, void *stream);
 /* Fully connected ReLU network with widths dims[0..n_layers] (no ReLU on the
  * output), He-normal initialisation from seed, zero biases. */
 int casl_mlp_create(const int32_t *dims, int n_layers, uint64_t seed,

@@ -93,6 +93,9 @@ SIGNATURES = {
                                         ctypes.c_void_p]),
     "casl_reinit_graph": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),
+    "casl_reinit_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                           ctypes.c_void_p]),
     "casl_reinit_masked": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                                           ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_void_p]),
     "casl_band_linf": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,

@@ -201,6 +201,23 @@ int casl_reinit_masked(casl_stencil_t *st, double *phi, const uint8_t *mask,
                        int n_iter, double limit_high, double limit_low,
                        int precision, void *stream);
 
+/* No counterpart in the reference: casl_reinit for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments (no mask) and the same
+ * results bit for bit; the call's buffers (the phi_0 copy, flags, subcell
+ * distances, stage buffers) come from one fixed device array of
+ * CASL_NOALLOC_SCRATCH_BYTES per GPU, which the library holds until the
+ * process ends, instead of the stencil's own scratch.  Calls that need more
+ * (66 B per node in 3D FP64 on the tiled path, 114 B on the two-pass
+ * path) return CASL_ERR_INVALID_ARGUMENT before anything runs.  The stencil
+ * itself is made beforehand (casl_stencil_create allocates).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other streams or
+ * host threads) race.  The first call on a device looks the array up there. */
+#define CASL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int casl_reinit_noalloc(casl_stencil_t *st, double *phi, int order_space,
+                        int order_pseudotime, int n_iter, double limit_high,
+                        double limit_low, int precision, void *stream);
+
 /* L-infinity norm of (a - b) over nodes with |ref| < band (FP64 reduction on
  * the device; used for the mixed-precision error report).  Result on host. */
 int casl_band_linf(casl_stencil_t *st, const double *a, const double *b,

@@ -166,6 +166,9 @@ SIGNATURES = {
     "casl_levelset_eval": (ctypes.c_int, [ctypes.POINTER(LevelSetSpec), ctypes.c_int, ctypes.c_int64,
                                           c_double_p, c_double_p]),
     "casl_forest_new": (ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_void_p)]),
+    "casl_forest_new_brick": (ctypes.c_int, [ctypes.c_int, c_double_p, c_double_p, c_int32_p, c_int32_p,
+                                             ctypes.POINTER(ctypes.c_void_p)]),
+    "casl_forest_trees": (ctypes.c_int, [ctypes.c_void_p, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),
     "casl_forest_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "casl_forest_refine_levelset": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                                    ctypes.c_double, ctypes.POINTER(LevelSetSpec)]),

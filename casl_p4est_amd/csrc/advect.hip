@@ -26,15 +26,15 @@ constexpr int BLOCK = 256;
 
 inline unsigned grid_for(int64_t n) { return unsigned((n + BLOCK - 1) / BLOCK); }
 
-template <int D>
-__global__ void __launch_bounds__(BLOCK) k_adv_dd(Tables<D> t, const double *__restrict__ f, double *__restrict__ fdd) {
+template <int D, class TB>
+__global__ void __launch_bounds__(BLOCK) k_adv_dd(TB t, const double *__restrict__ f, double *__restrict__ fdd) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n < t.N) node_dd<D, double>(t, n, f, fdd);
 }
 
 // out = stage, or (RK_AVG) out = (out + stage) / 2 with out holding phi_n
-template <int D, bool RK_AVG>
-__global__ void __launch_bounds__(BLOCK) k_advect(Tables<D> t, const double *__restrict__ vn, double dt,
+template <int D, bool RK_AVG, class TB>
+__global__ void __launch_bounds__(BLOCK) k_advect(TB t, const double *__restrict__ vn, double dt,
                                                   const double *__restrict__ phin, const double *__restrict__ dd,
                                                   double *out) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -117,8 +117,8 @@ __global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_pe
 
 // min over nodes of 0.8 |s_min / vn|, s_min the shortest |arm|; non-negative
 // doubles order like their bit patterns, so an unsigned atomicMin reduces them
-template <int D>
-__global__ void __launch_bounds__(BLOCK) k_cfl(Tables<D> t, const double *__restrict__ vn,
+template <int D, class TB>
+__global__ void __launch_bounds__(BLOCK) k_cfl(TB t, const double *__restrict__ vn,
                                                unsigned long long *__restrict__ result) {
   __shared__ double red[BLOCK];
   double m = DBL_MAX;
@@ -142,8 +142,7 @@ __global__ void k_set_u64(unsigned long long *p, unsigned long long v) { *p = v;
 template <int D>
 void run_advect(casl_stencil *st, double *phi, const double *vn, const double *vnp1, double dt, hipStream_t s) {
   const int64_t N = st->N;
-  const Tables<D> t = st->tables<D>();
-  if (st->tiled && st->path == CASL_PATH_TILED) {
+  if (st->use_tiles(true)) {
     st->phi0.ensure(N);
     TileArgs<double> ta = tile_args<double>(st);
     const dim3 g(unsigned(st->n_tiles)), b(TILE_BLOCK_F64);
@@ -165,19 +164,23 @@ void run_advect(casl_stencil *st, double *phi, const double *vn, const double *v
   double *phin = st->phi0.p, *dd = st->ddn64.p;
   const unsigned g = grid_for(N);
   HIP_CHECK(hipMemcpyAsync(phin, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
-  prof_begin("k_adv_dd", s);
-  hipLaunchKernelGGL(k_adv_dd<D>, dim3(g), dim3(BLOCK), 0, s, t, phin, dd);
-  prof_end("k_adv_dd", s);
-  prof_begin("k_advect", s);
-  hipLaunchKernelGGL((k_advect<D, false>), dim3(g), dim3(BLOCK), 0, s, t, vn, dt, phin, dd, phi);
-  prof_end("k_advect", s);
-  prof_begin("k_adv_dd", s);
-  hipLaunchKernelGGL(k_adv_dd<D>, dim3(g), dim3(BLOCK), 0, s, t, phi, dd);
-  prof_end("k_adv_dd", s);
-  // phin <- (phi_n + stage(phi)) / 2, then back into phi
-  prof_begin("k_advect", s);
-  hipLaunchKernelGGL((k_advect<D, true>), dim3(g), dim3(BLOCK), 0, s, t, vnp1 ? vnp1 : vn, dt, phi, dd, phin);
-  prof_end("k_advect", s);
+  st->with_tables<D>(true, [&](auto t) {
+    using TB = decltype(t);
+    prof_begin("k_adv_dd", s);
+    hipLaunchKernelGGL((k_adv_dd<D, TB>), dim3(g), dim3(BLOCK), 0, s, t, phin, dd);
+    prof_end("k_adv_dd", s);
+    prof_begin("k_advect", s);
+    hipLaunchKernelGGL((k_advect<D, false, TB>), dim3(g), dim3(BLOCK), 0, s, t, vn, dt, phin, dd, phi);
+    prof_end("k_advect", s);
+    prof_begin("k_adv_dd", s);
+    hipLaunchKernelGGL((k_adv_dd<D, TB>), dim3(g), dim3(BLOCK), 0, s, t, phi, dd);
+    prof_end("k_adv_dd", s);
+    // phin <- (phi_n + stage(phi)) / 2, then back into phi
+    prof_begin("k_advect", s);
+    hipLaunchKernelGGL((k_advect<D, true, TB>), dim3(g), dim3(BLOCK), 0, s, t, vnp1 ? vnp1 : vn, dt, phi, dd,
+                       phin);
+    prof_end("k_advect", s);
+  });
   HIP_CHECK(hipGetLastError());
   HIP_CHECK(hipMemcpyAsync(phi, phin, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
 }
@@ -210,9 +213,13 @@ int casl_advect_cfl_dt(casl_stencil_t *st, const double *vn, double *dt, void *s
     hipLaunchKernelGGL(k_set_u64, dim3(1), dim3(1), 0, s, st->red.p, init);
     const unsigned g = std::min<unsigned>(grid_for(st->N), 2048u);
     if (st->dim == 2)
-      hipLaunchKernelGGL(k_cfl<2>, dim3(g), dim3(BLOCK), 0, s, st->tables<2>(), vn, st->red.p);
+      st->with_tables<2>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_cfl<2, decltype(t)>), dim3(g), dim3(BLOCK), 0, s, t, vn, st->red.p);
+      });
     else
-      hipLaunchKernelGGL(k_cfl<3>, dim3(g), dim3(BLOCK), 0, s, st->tables<3>(), vn, st->red.p);
+      st->with_tables<3>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_cfl<3, decltype(t)>), dim3(g), dim3(BLOCK), 0, s, t, vn, st->red.p);
+      });
     HIP_CHECK(hipGetLastError());
     unsigned long long bits = 0;
     HIP_CHECK(hipMemcpyAsync(&bits, st->red.p, sizeof(bits), hipMemcpyDeviceToHost, s));

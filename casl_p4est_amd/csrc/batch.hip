@@ -437,6 +437,11 @@ void build_batch_tiles(casl_batch *B, const casl_flat_stencil *trees, const std:
   const int64_t N = off[size_t(nt)];
   validate_tile_host(all, D, N);
   upload_tile_host(all, N, &B->st);
+  // k_batch_tiles keeps the dyadic shortcuts in both precisions
+  if (!B->st.dyadic_geometry()) {
+    B->tile_error = "the trees' coarsest cells are not powers of two";
+    return;
+  }
   B->tree_tile.ensure(tree_tile.size());
   HIP_CHECK(hipMemcpy(B->tree_tile.p, tree_tile.data(), tree_tile.size() * sizeof(int32_t), hipMemcpyHostToDevice));
   B->tiled = true;
@@ -513,7 +518,7 @@ int casl_batch_create(const casl_flat_stencil *trees, int32_t n_trees, casl_batc
     all.d_tr = dtr.data();
     all.wall = wall.data();
     all.zero_distance_threshold = zdt;
-    upload_tables(&all, &B->st, false);
+    upload_tables(&all, &B->st, false, false);  // the batch kernels read FP32 distances only
     B->n_trees = n_trees;
     B->max_tree = max_tree;
     std::vector<int32_t> order(n_trees);

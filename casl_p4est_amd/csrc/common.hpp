@@ -4,6 +4,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <string>
+#include <exception>
 #include <thread>
 #include <vector>
 #include <algorithm>
@@ -27,13 +28,24 @@ void parallel_for(int64_t n, Fn fn, int64_t min_chunk = 4096) {
     if (n > 0) fn(int64_t(0), n);
     return;
   }
+  // an exception in a worker is kept and rethrown here after every worker
+  // has joined (one escaping a thread would end the process)
+  std::vector<std::exception_ptr> err(size_t(chunks), nullptr);
   std::vector<std::thread> pool;
   pool.reserve(chunks);
   for (int64_t c = 0; c < chunks; ++c) {
     int64_t b = n * c / chunks, e = n * (c + 1) / chunks;
-    pool.emplace_back([=, &fn]() { fn(b, e); });
+    pool.emplace_back([=, &fn, &err]() {
+      try {
+        fn(b, e);
+      } catch (...) {
+        err[size_t(c)] = std::current_exception();
+      }
+    });
   }
   for (auto &th : pool) th.join();
+  for (auto &x : err)
+    if (x) std::rethrow_exception(x);
 }
 
 // Morton (z-order) keys.  Bit b of axis a lands at bit D*b + a, so at equal

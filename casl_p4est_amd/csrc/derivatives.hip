@@ -21,8 +21,8 @@ namespace {
 
 constexpr int BLOCK = 256;
 
-template <int D>
-__global__ void __launch_bounds__(BLOCK) k_gradient(Tables<D> t, const double *__restrict__ f, double *__restrict__ fd) {
+template <int D, class TB>
+__global__ void __launch_bounds__(BLOCK) k_gradient(TB t, const double *__restrict__ f, double *__restrict__ fd) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
   double g[D];
@@ -31,8 +31,8 @@ __global__ void __launch_bounds__(BLOCK) k_gradient(Tables<D> t, const double *_
   for (int a = 0; a < D; ++a) fd[int64_t(a) * t.N + n] = g[a];
 }
 
-template <int D>
-__global__ void __launch_bounds__(BLOCK) k_normals(Tables<D> t, const double *__restrict__ phi, double *__restrict__ nrm) {
+template <int D, class TB>
+__global__ void __launch_bounds__(BLOCK) k_normals(TB t, const double *__restrict__ phi, double *__restrict__ nrm) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
   double g[D];
@@ -45,8 +45,8 @@ __global__ void __launch_bounds__(BLOCK) k_normals(Tables<D> t, const double *__
 }
 
 // kappa = sum_a d(n_a)/dx_a, each with its own transverse corrections
-template <int D>
-__global__ void __launch_bounds__(BLOCK) k_curvature(Tables<D> t, const double *__restrict__ nrm, double *__restrict__ kap) {
+template <int D, class TB>
+__global__ void __launch_bounds__(BLOCK) k_curvature(TB t, const double *__restrict__ nrm, double *__restrict__ kap) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
   double k = 0.0;
@@ -74,9 +74,13 @@ int casl_first_derivatives(casl_stencil_t *st, const double *f, double *fd, void
     check_stencil(st, f, fd);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (st->dim == 2)
-      hipLaunchKernelGGL(k_gradient<2>, dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<2>(), f, fd);
+      st->with_tables<2>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_gradient<2, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, f, fd);
+      });
     else
-      hipLaunchKernelGGL(k_gradient<3>, dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<3>(), f, fd);
+      st->with_tables<3>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_gradient<3, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, f, fd);
+      });
     HIP_CHECK(hipGetLastError());
   });
 }
@@ -86,9 +90,13 @@ int casl_normals(casl_stencil_t *st, const double *phi, double *normals, void *s
     check_stencil(st, phi, normals);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (st->dim == 2)
-      hipLaunchKernelGGL(k_normals<2>, dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<2>(), phi, normals);
+      st->with_tables<2>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_normals<2, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, phi, normals);
+      });
     else
-      hipLaunchKernelGGL(k_normals<3>, dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<3>(), phi, normals);
+      st->with_tables<3>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_normals<3, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, phi, normals);
+      });
     HIP_CHECK(hipGetLastError());
   });
 }
@@ -98,9 +106,13 @@ int casl_mean_curvature(casl_stencil_t *st, const double *normals, double *kappa
     check_stencil(st, normals, kappa);
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (st->dim == 2)
-      hipLaunchKernelGGL(k_curvature<2>, dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<2>(), normals, kappa);
+      st->with_tables<2>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_curvature<2, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, normals, kappa);
+      });
     else
-      hipLaunchKernelGGL(k_curvature<3>, dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<3>(), normals, kappa);
+      st->with_tables<3>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_curvature<3, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, normals, kappa);
+      });
     HIP_CHECK(hipGetLastError());
   });
 }

@@ -39,16 +39,16 @@ constexpr int BLOCK = 256;
 
 inline unsigned grid_for(int64_t n, int block = BLOCK) { return unsigned((n + block - 1) / block); }
 
-template <int D, class R>
-__global__ void __launch_bounds__(BLOCK) k_dd(Tables<D> t, const R *__restrict__ f, R *__restrict__ fdd) {
+template <int D, class R, class TB>
+__global__ void __launch_bounds__(BLOCK) k_dd(TB t, const R *__restrict__ f, R *__restrict__ fdd) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
   node_dd<D, R>(t, n, f, fdd);
 }
 
-template <int D, class S>
+template <int D, class S, class TB>
 __global__ void __launch_bounds__(BLOCK)
-    k_prep(Tables<D> t, const double *__restrict__ phi0, const double *__restrict__ dd0,
+    k_prep(TB t, const double *__restrict__ phi0, const double *__restrict__ dd0,
            const uint8_t *__restrict__ mask, double lim_low, double lim_high, double zdt, int second_order,
            uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -59,9 +59,9 @@ __global__ void __launch_bounds__(BLOCK)
 // Pass B of one pseudo-time stage.  out[n] = result, or for the second RK2
 // stage out[n] = 0.5*(out[n] + result): out then holds phi_n, each node reads
 // and writes only its own entry, and phin is a different buffer.
-template <int D, class R, bool SECOND_ORDER, bool RK_AVG>
+template <int D, class R, bool SECOND_ORDER, bool RK_AVG, class TB>
 __global__ void __launch_bounds__(BLOCK)
-    k_stage(Tables<D> t, const uint16_t *__restrict__ flags, const R *__restrict__ s_fix,
+    k_stage(TB t, const uint16_t *__restrict__ flags, const R *__restrict__ s_fix,
             const double *__restrict__ phi0, const R *__restrict__ phin, const R *__restrict__ dd, R *out) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
@@ -170,18 +170,26 @@ dev::TileArgs<R> tile_args(const casl_stencil *st) {
   ta.urec = st->t_urec.p;
   ta.urec_start = st->t_urec_start.p;
   ta.nuni = st->t_nuni.p;
-  ta.geom.pow2 = 1;
+  ta.geom.pow2 = st->dyadic_geometry() ? 1 : 0;
   for (int a = 0; a < 3; ++a) {
     ta.geom.base[a] = R(st->geom_base[a]);
     ta.geom.unit[a] = R(st->geom_unit[a]);
-    int e = 0;
-    const double m = std::frexp(st->geom_base[a], &e);
     ta.geom.inv_base[a] = R(1.0 / st->geom_base[a]);
-    // exponent arithmetic needs a normal base whose arm cells (down to 2^-15
-    // of it) and their inverses stay normal: true for any real tree
-    if (m != 0.5 || e < -90 || e > 90) ta.geom.pow2 = 0;
     ta.geom.base_bits[a] = exp_word(ta.geom.base[a]);
     ta.geom.inv_bits[a] = exp_word(ta.geom.inv_base[a]);
+  }
+  ta.geom.cubic = 1;
+  for (int a = 1; a < st->dim; ++a) ta.geom.cubic &= int(st->geom_base[a] == st->geom_base[0]);
+  ta.geom.exp_pack = 0;
+  ta.geom.lmax = 0;
+  if (ta.geom.pow2) {
+    int e = 0;
+    for (int a = 0; a < 3; ++a) {
+      std::frexp(st->geom_base[a], &e);  // base = 0.5 * 2^e
+      ta.geom.exp_pack |= uint32_t((e - 1 + 128) & 255) << (8 * a);
+    }
+    std::frexp(st->geom_base[0] / st->geom_unit[0], &e);  // base / unit = 2^lmax
+    ta.geom.lmax = e - 1;
   }
   ta.N = st->N;
   return ta;
@@ -264,17 +272,18 @@ template <int D, class R>
 void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_space, int order_time, int n_iter,
                 double limit_high, double limit_low, hipStream_t s, const ReinitBufs<R> &B) {
   const int64_t N = st->N;
-  const Tables<D> t = st->tables<D>();
   double lim_low, lim_high;
   widened_limits(limit_high, limit_low, st->zdt, lim_low, lim_high);
   const bool so = order_space > 1;
   const unsigned grid = grid_for(N);
 
-  const bool tiled = st->tiled && st->path == CASL_PATH_TILED;
+  const bool tiled = st->use_tiles(sizeof(R) == 8);
   HIP_CHECK(hipMemcpyAsync(B.phi0, phi, sizeof(double) * N, hipMemcpyDeviceToDevice, s));
   if (so && !tiled) {
     prof_begin("k_dd", s);
-    hipLaunchKernelGGL((k_dd<D, double>), dim3(grid), dim3(BLOCK), 0, s, t, B.phi0, B.dd0);
+    st->with_tables<D>(true, [&](auto tt) {
+      hipLaunchKernelGGL((k_dd<D, double, decltype(tt)>), dim3(grid), dim3(BLOCK), 0, s, tt, B.phi0, B.dd0);
+    });
     prof_end("k_dd", s);
     HIP_CHECK(hipGetLastError());
   }
@@ -301,33 +310,42 @@ void run_reinit(casl_stencil *st, double *phi, const uint8_t *mask, int order_sp
     prof_end("k_tile_prep", s);
   } else {
     prof_begin("k_prep", s);
-    hipLaunchKernelGGL((k_prep<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, B.phi0, B.dd0, mask, lim_low,
-                       lim_high, st->zdt, int(so), B.flags, sfix);
+    st->with_tables<D>(true, [&](auto tt) {
+      hipLaunchKernelGGL((k_prep<D, R, decltype(tt)>), dim3(grid), dim3(BLOCK), 0, s, tt, B.phi0, B.dd0, mask,
+                         lim_low, lim_high, st->zdt, int(so), B.flags, sfix);
+    });
     prof_end("k_prep", s);
   }
   HIP_CHECK(hipGetLastError());
 
+  // the FP64 stages read the FP64 distances when the stencil has them; the
+  // mixed path's FP32 stages read the FP32 tables
   auto dd_pass = [&](const R *in) {
     prof_begin("k_dd", s);
-    hipLaunchKernelGGL((k_dd<D, R>), dim3(grid), dim3(BLOCK), 0, s, t, in, ddn);
+    st->with_tables<D>(sizeof(R) == 8, [&](auto tt) {
+      hipLaunchKernelGGL((k_dd<D, R, decltype(tt)>), dim3(grid), dim3(BLOCK), 0, s, tt, in, ddn);
+    });
     prof_end("k_dd", s);
   };
   auto stage_launch = [&](const R *in, R *out, bool avg) {
-    if (so) {
-      if (avg)
-        hipLaunchKernelGGL((k_stage<D, R, true, true>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0, in,
-                           ddn, out);
-      else
-        hipLaunchKernelGGL((k_stage<D, R, true, false>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0,
-                           in, ddn, out);
-    } else {
-      if (avg)
-        hipLaunchKernelGGL((k_stage<D, R, false, true>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0,
-                           in, ddn, out);
-      else
-        hipLaunchKernelGGL((k_stage<D, R, false, false>), dim3(grid), dim3(BLOCK), 0, s, t, B.flags, sfix, B.phi0,
-                           in, ddn, out);
-    }
+    st->with_tables<D>(sizeof(R) == 8, [&](auto tt) {
+      using TB = decltype(tt);
+      if (so) {
+        if (avg)
+          hipLaunchKernelGGL((k_stage<D, R, true, true, TB>), dim3(grid), dim3(BLOCK), 0, s, tt, B.flags, sfix,
+                             B.phi0, in, ddn, out);
+        else
+          hipLaunchKernelGGL((k_stage<D, R, true, false, TB>), dim3(grid), dim3(BLOCK), 0, s, tt, B.flags, sfix,
+                             B.phi0, in, ddn, out);
+      } else {
+        if (avg)
+          hipLaunchKernelGGL((k_stage<D, R, false, true, TB>), dim3(grid), dim3(BLOCK), 0, s, tt, B.flags, sfix,
+                             B.phi0, in, ddn, out);
+        else
+          hipLaunchKernelGGL((k_stage<D, R, false, false, TB>), dim3(grid), dim3(BLOCK), 0, s, tt, B.flags, sfix,
+                             B.phi0, in, ddn, out);
+      }
+    });
   };
   auto stage = [&](const R *in, R *out, bool avg) {
     prof_begin("k_stage", s);
@@ -546,7 +564,7 @@ void upload_tile_tables(const casl_flat_stencil *tab, casl_stencil *st) {
   st->path = st->n_tiles >= int64_t(prop.multiProcessorCount) ? CASL_PATH_TILED : CASL_PATH_TWO_PASS;
 }
 
-void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles) {
+void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles, bool allow_fp64_copy) {
   validate_tables(tab);
   require_device();
   const int D = tab->dim;
@@ -557,16 +575,20 @@ void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_til
     if (tab->nbr[i] < 0 || tab->nbr[i] >= N)
       throw std::runtime_error("stencil node index out of range at entry " + std::to_string(i));
   const size_t nd = size_t(F) * N, nt = size_t(F) * T * N;
+  // FP32 distances for every kernel; when FP32 does not hold them exactly (a
+  // box whose edges are not dyadic) the FP64 kernels read an FP64 copy
   std::vector<float> dist(nd + nt);
+  bool exact = true;
   for (size_t i = 0; i < nd; ++i) {
     dist[i] = float(tab->d[i]);
-    if (double(dist[i]) != tab->d[i]) throw std::domain_error("face distance not exact in FP32");
+    exact = exact && double(dist[i]) == tab->d[i];
     if (!(tab->d[i] != 0.0)) throw std::runtime_error("zero face distance");
   }
   for (size_t i = 0; i < nt; ++i) {
     dist[nd + i] = float(tab->d_tr[i]);
-    if (double(dist[nd + i]) != tab->d_tr[i]) throw std::domain_error("transverse distance not exact in FP32");
+    exact = exact && double(dist[nd + i]) == tab->d_tr[i];
   }
+  if (!exact && !allow_fp64_copy) throw std::domain_error("stencil distance not exact in FP32");
   st->dim = D;
   st->N = N;
   st->zdt = tab->zero_distance_threshold;
@@ -576,6 +598,12 @@ void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_til
   st->wall.ensure(N);
   HIP_CHECK(hipMemcpy(st->nbr.p, tab->nbr, nidx * sizeof(int32_t), hipMemcpyHostToDevice));
   HIP_CHECK(hipMemcpy(st->dist.p, dist.data(), dist.size() * sizeof(float), hipMemcpyHostToDevice));
+  st->wide = !exact;
+  if (!exact) {
+    st->dist64.ensure(nd + nt);
+    HIP_CHECK(hipMemcpy(st->dist64.p, tab->d, nd * sizeof(double), hipMemcpyHostToDevice));
+    HIP_CHECK(hipMemcpy(st->dist64.p + nd, tab->d_tr, nt * sizeof(double), hipMemcpyHostToDevice));
+  }
   HIP_CHECK(hipMemcpy(st->wall.p, tab->wall, N * sizeof(uint8_t), hipMemcpyHostToDevice));
   if (with_tiles) upload_tile_tables(tab, st);
 }
@@ -676,7 +704,7 @@ int casl_stencil_create(const casl_flat_stencil *tab, casl_stencil_t **out) {
     if (!out) throw std::invalid_argument("null out");
     *out = nullptr;
     auto st = std::make_unique<casl_stencil>();
-    upload_tables(tab, st.get(), true);
+    upload_tables(tab, st.get(), true, true);
     *out = st.release();
   });
 }
@@ -768,11 +796,15 @@ int casl_stencil_tile_check(const casl_stencil_t *st, const casl_flat_stencil *t
     };
     const int64_t nt = int64_t(H.tile_start.size()) - 1;
     if (st->n_tiles != nt || st->n_halo != int64_t(H.halo_idx.size()) ||
-        st->n_records != int64_t(H.rec.size()) + int64_t(H.urec_start.back()) || st->n_irregular != H.n_irregular ||
-        st->geom_base[0] != H.size[0] || st->geom_unit[0] != H.size[0] * std::ldexp(1.0, -H.lmax)) {
-      set_error("tiled encodings differ in their counts or scales");
+        st->n_records != int64_t(H.rec.size()) + int64_t(H.urec_start.back()) || st->n_irregular != H.n_irregular) {
+      set_error("tiled encodings differ in their counts");
       return;
     }
+    for (int a = 0; a < 3; ++a)
+      if (st->geom_base[a] != H.size[a] || st->geom_unit[a] != H.size[a] * std::ldexp(1.0, -H.lmax)) {
+        set_error("tiled encodings differ in their scales");
+        return;
+      }
     const bool ok =
         same("tile_start", st->t_tile_start.p, H.tile_start.data(), H.tile_start.size() * sizeof(int32_t)) &&
         same("halo_start", st->t_halo_start.p, H.halo_start.data(), H.halo_start.size() * sizeof(int32_t)) &&
@@ -804,9 +836,13 @@ int casl_second_derivatives(casl_stencil_t *st, const double *f, double *fdd, vo
     if (!st || !f || !fdd) throw std::invalid_argument("null argument");
     hipStream_t s = static_cast<hipStream_t>(stream);
     if (st->dim == 2)
-      hipLaunchKernelGGL((k_dd<2, double>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<2>(), f, fdd);
+      st->with_tables<2>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_dd<2, double, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, f, fdd);
+      });
     else
-      hipLaunchKernelGGL((k_dd<3, double>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, st->tables<3>(), f, fdd);
+      st->with_tables<3>(true, [&](auto t) {
+        hipLaunchKernelGGL((k_dd<3, double, decltype(t)>), dim3(grid_for(st->N)), dim3(BLOCK), 0, s, t, f, fdd);
+      });
     HIP_CHECK(hipGetLastError());
   });
 }
@@ -831,7 +867,7 @@ template <int D, class R>
 static void reinit_on_stencil(casl_stencil_t *st, double *phi, const uint8_t *mask, int order_space,
                               int order_time, int n_iter, double limit_high, double limit_low, hipStream_t s,
                               bool prepare_only) {
-  const bool so = order_space > 1, tiled = st->tiled && st->path == CASL_PATH_TILED;
+  const bool so = order_space > 1, tiled = st->use_tiles(sizeof(R) == 8);
   const ReinitBufs<R> B = reinit_scratch<D, R>(st, so, tiled);
   if (!prepare_only) run_reinit<D, R>(st, phi, mask, order_space, order_time, n_iter, limit_high, limit_low, s, B);
 }
@@ -863,7 +899,7 @@ static void dispatch_reinit(casl_stencil_t *st, double *phi, const uint8_t *mask
 template <int D, class R>
 static void reinit_in_buffer_t(casl_stencil_t *st, double *phi, int order_space, int order_time, int n_iter,
                                double limit_high, double limit_low, hipStream_t s, unsigned char *base, size_t cap) {
-  const bool so = order_space > 1, tiled = st->tiled && st->path == CASL_PATH_TILED;
+  const bool so = order_space > 1, tiled = st->use_tiles(sizeof(R) == 8);
   const ReinitSizes z = reinit_sizes<D, R>(size_t(st->N), so, tiled);
   const size_t bytes[7] = {z.phi0 * sizeof(double), z.flags * sizeof(uint16_t), z.dd0 * sizeof(double),
                            z.sfix * sizeof(R), z.work * sizeof(R), z.tmp * sizeof(R), z.ddn * sizeof(R)};

@@ -132,8 +132,8 @@ namespace dev {
 enum : uint16_t { MODE_UPDATE = 0, MODE_ANCHOR = 1, MODE_COPY0 = 2, MODE_MASKED = 3 };
 
 // Pass A at one node: second_derivatives_central (laplace) of f.
-template <int D, class R>
-__device__ __forceinline__ void node_dd(const Tables<D> &t, int64_t n, const R *__restrict__ f, R *__restrict__ fdd) {
+template <int D, class R, class TB>
+__device__ __forceinline__ void node_dd(const TB &t, int64_t n, const R *__restrict__ f, R *__restrict__ fdd) {
   R f000, fnb[2 * D], dd[D];
   laplace<D, R>(t, n, f, f000, fnb, dd);
 #pragma unroll
@@ -199,8 +199,8 @@ __device__ __forceinline__ uint16_t prep_value(const TB &t, int64_t j, bool mask
 }
 
 // prep_value on the grid-wide tables, flags and s_fix in node order
-template <int D, class S>
-__device__ __forceinline__ void node_prep(const Tables<D> &t, int64_t n, const double *__restrict__ phi0,
+template <int D, class S, class TB>
+__device__ __forceinline__ void node_prep(const TB &t, int64_t n, const double *__restrict__ phi0,
                                           const double *__restrict__ dd0, const uint8_t *__restrict__ mask,
                                           double lim_low, double lim_high, double zdt, bool second_order,
                                           uint16_t *__restrict__ flags, S *__restrict__ s_fix) {
@@ -392,8 +392,8 @@ __device__ __forceinline__ double advect_value(const TB &t, int64_t j, uint8_t w
 }
 
 // stage_value on the grid-wide tables
-template <int D, class R, bool SECOND_ORDER>
-__device__ __forceinline__ R node_stage(const Tables<D> &t, int64_t n, uint16_t fl, const R *__restrict__ s_fix,
+template <int D, class R, bool SECOND_ORDER, class TB>
+__device__ __forceinline__ R node_stage(const TB &t, int64_t n, uint16_t fl, const R *__restrict__ s_fix,
                                         const double *__restrict__ phi0, const R *__restrict__ phin,
                                         const R *__restrict__ dd) {
   return stage_value<D, R, SECOND_ORDER>(t, n, fl, t.wall[n], s_fix + n, t.N, phi0 + n, phin, dd, t.N);
@@ -410,6 +410,10 @@ struct casl_stencil {
   double zdt = 0;
   casl::DevBuf<int32_t> nbr;
   casl::DevBuf<float> dist;
+  // FP64 copy of the distances when FP32 does not hold them exactly (a box
+  // whose edges are not dyadic): the FP64 grid-wide kernels read it
+  casl::DevBuf<double> dist64;
+  bool wide = false;
   casl::DevBuf<uint8_t> wall;
   // scratch, grown on demand
   casl::DevBuf<double> phi0, dd0, tmp64, ddn64, sfix64;
@@ -467,12 +471,45 @@ struct casl_stencil {
     t.N = N;
     return t;
   }
+  template <int D>
+  casl::dev::Tables<D, double> tables64() const {
+    casl::dev::Tables<D, double> t;
+    t.nbr = nbr.p;
+    t.dist = dist64.p;
+    t.wall = wall.p;
+    t.N = N;
+    return t;
+  }
+  // fn(tables) with the tables an FP64 kernel reads: the FP64 copy when
+  // there is one, else the FP32 tables (exact then)
+  template <int D, class Fn>
+  void with_tables(bool fp64, Fn &&fn) const {
+    if (fp64 && wide)
+      fn(tables64<D>());
+    else
+      fn(tables<D>());
+  }
+
+  // Every tile scale is a normal power of two whose arm cells (down to 2^-15
+  // of it) and their inverses stay normal: true for a tree of any
+  // power-of-two edge.  The tiled kernels' shortcuts are exact then.
+  bool dyadic_geometry() const { return casl::dyadic_scales(geom_base, 3); }
+  // The tiled kernels run this call.  They take a regular face's value as
+  // its one corner's value, which equals the reference's weighted form
+  // (g w) / w only when w, a product of transverse extents, is a power of
+  // two: FP64 on any other geometry runs the grid-wide kernels, which keep
+  // the weighted form.
+  bool use_tiles(bool fp64) const {
+    return tiled && path == CASL_PATH_TILED && (!fp64 || dyadic_geometry());
+  }
 };
 
 namespace casl {
 // Validates and uploads host tables into st (shared by stencils and batches).
-// with_tiles also builds the tiled encoding for the fused stage kernel.
-void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles);
+// with_tiles also builds the tiled encoding for the fused stage kernel;
+// allow_fp64_copy keeps an FP64 copy of distances FP32 does not hold exactly
+// (else such tables are refused).
+void upload_tables(const casl_flat_stencil *tab, casl_stencil *st, bool with_tiles, bool allow_fp64_copy);
 // Bounds check of a tiled encoding of n_nodes nodes (throws), and its upload
 // into st (tile arrays, one geometry).
 void validate_tile_host(const TileHost &H, int D, int64_t n_nodes);

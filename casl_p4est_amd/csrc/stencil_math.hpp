@@ -15,8 +15,9 @@ namespace dev {
 constexpr double EPS = 1e-13;  // src/casl_math.h:23
 
 // Device view of the flattened stencils of casl_flat_stencil.  Distances are
-// stored as FP32 (exact for dyadic trees; checked at upload).
-template <int D>
+// stored as DT: FP32 (exact for dyadic trees), or FP64 for the FP64 kernels
+// of a grid whose distances FP32 does not hold exactly.
+template <int D, class DT = float>
 struct Tables {
   static constexpr bool REGULAR = false;  // see the REGULAR shortcuts below
   static constexpr bool FASTDIV = false;  // see second_difference
@@ -26,13 +27,13 @@ struct Tables {
   static constexpr int C = 1 << (D - 1);
   static constexpr int T = 2 * (D - 1);
   const int32_t *__restrict__ nbr;  // [F*C][N]
-  const float *__restrict__ dist;   // [F][N] face distances, then [F*T][N] transverse
+  const DT *__restrict__ dist;      // [F][N] face distances, then [F*T][N] transverse
   const uint8_t *__restrict__ wall; // [N]
   int64_t N;
 
   __device__ __forceinline__ int32_t idx(int f, int c, int64_t n) const { return nbr[(int64_t(f) * C + c) * N + n]; }
-  __device__ __forceinline__ float d(int f, int64_t n) const { return dist[int64_t(f) * N + n]; }
-  __device__ __forceinline__ float dt(int f, int k, int64_t n) const {
+  __device__ __forceinline__ DT d(int f, int64_t n) const { return dist[int64_t(f) * N + n]; }
+  __device__ __forceinline__ DT dt(int f, int k, int64_t n) const {
     return dist[(int64_t(F) + int64_t(f) * T + k) * N + n];
   }
 };
@@ -116,8 +117,8 @@ __device__ __forceinline__ float qfma<float>(float a, float b, float c) {
 }
 
 // Exact-inverse division.  A stencil accessor with TB::FASTDIV and t.fast
-// has power-of-two face distances d_f = 1 / iq[f] (cubic dyadic cells of a
-// power-of-two tree, no wall arm).  Dividing by a power of two and
+// has power-of-two face distances d_f = 1 / iq[f] (dyadic cells of a tree
+// whose edges are powers of two, no wall arm).  Dividing by a power of two and
 // multiplying by its exactly representable inverse give the same correctly
 // rounded result, so these helpers return bit-for-bit what the plain
 // division returns, without its instruction sequence.
@@ -162,18 +163,18 @@ __device__ __forceinline__ R div_dsum(const TB &t, R x, int a, int64_t n) {
 // the full expressions for finite inputs (at most the sign of a zero differs).
 template <int D, class R, class G, class TB>
 __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G *__restrict__ g) {
-  if constexpr (TB::REGULAR) return R(g[t.idx(f, 0, n)]);
   const R d_m0 = R(t.dt(f, 0, n)), d_p0 = R(t.dt(f, 1, n));
-  if constexpr (D == 2) {
+  if constexpr (TB::REGULAR) {
+    return R(g[t.idx(f, 0, n)]);
+  } else if constexpr (D == 2) {
     const R w_mm = d_p0;
     const R w_pm = d_m0;
     const R normalization = (d_m0 + d_p0);
     const R num = qfma(R(g[t.idx(f, 0, n)]), w_mm, R(g[t.idx(f, 1, n)]) * w_pm);
-    // d_m0 + d_p0 is the arm cell's transverse size, which equals d_f: the
-    // tiled encoding exists only for grids of square cells (the tile builder
-    // rejects other boxes, which then take the grid-wide tables)
+    // d_m0 + d_p0 is the arm cell's transverse size, a power of two on a
+    // dyadic geometry: its exact inverse (inv_norm) replaces the division
     if constexpr (irregular_fastdiv<R, TB>()) {
-      if (t.fast) return num * t.iq[f];
+      if (t.fast) return num * t.inv_norm(f);
     }
     return qdiv(num, normalization);
   } else {
@@ -185,10 +186,10 @@ __device__ __forceinline__ R face_linear(const TB &t, int f, int64_t n, const G 
     const R normalization = (d_m0 + d_p0) * (d_0m + d_0p);
     const R num = qfma(R(g[t.idx(f, 0, n)]), w_mm,
                        qfma(R(g[t.idx(f, 2, n)]), w_mp, qfma(R(g[t.idx(f, 1, n)]), w_pm, R(g[t.idx(f, 3, n)]) * w_pp)));
-    // both transverse sums are the arm cell's size d_f (tiled grids have cubic
-    // cells, as in 2D)
+    // the transverse sums are the arm cell's transverse sizes, powers of two
+    // on a dyadic geometry (inv_norm: the inverse of their product)
     if constexpr (irregular_fastdiv<R, TB>()) {
-      if (t.fast) return num * (t.iq[f] * t.iq[f]);
+      if (t.fast) return num * t.inv_norm(f);
     }
     return qdiv(num, normalization);
   }
@@ -206,15 +207,16 @@ __device__ __forceinline__ R central_second_derivative(R fp, R f0, R fm, R dp, R
 // the division by d_p + d_m one by 1/(2 d_p) when d_p == d_m: the same
 // correctly rounded results, without the division sequences.
 //
-// TB::UNIFORM: every arm has the same power-of-two length q (a uniform node of
-// the tiled encoding on a dyadic geometry).  Then (fp - f0) * (1/q) and
-// (fm - f0) * (1/q) are exact scalings, their correctly rounded sum is the
-// rounded sum of the differences times 1/q, and the factors 2 and 1/(2q)
-// are exact too: ((fp - f0) + (fm - f0)) * (1/q)^2 gives the same bits as the
-// path above, in FP64 and (with its fused first product) in FP32.
+// TB::UNIFORM: both arms of axis a have the same power-of-two length q (a
+// uniform node of the tiled encoding on a dyadic geometry).  Then
+// (fp - f0) * (1/q) and (fm - f0) * (1/q) are exact scalings, their correctly
+// rounded sum is the rounded sum of the differences times 1/q, and the
+// factors 2 and 1/(2q) are exact too: ((fp - f0) + (fm - f0)) * (1/q)^2 gives
+// the same bits as the path below, in FP64 and (with its fused first product)
+// in FP32.
 template <class R, class TB>
 __device__ __forceinline__ R second_difference(const TB &t, int a, R fp, R f0, R fm, int64_t n) {
-  if constexpr (TB::UNIFORM) return ((fp - f0) + (fm - f0)) * t.iq2;
+  if constexpr (TB::UNIFORM) return ((fp - f0) + (fm - f0)) * t.iq2[a];
   const int fpp = 2 * a + 1, fmm = 2 * a;
   if constexpr (TB::FASTDIV) {
     if (t.fast) {
@@ -359,8 +361,8 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
 // ------------------------------------------------- hanging faces by slot
 // A stencil with TB::SLOTS (a node of the tiled encoding with hanging arms)
 // names the faces that hang: t.hface(q) for q < TB::NH (HANG_NONE past the
-// last one), with corners t.hidx(q, c) and transverse offsets
-// t.hdt(q, k, q_f) (k: m0, p0, 0m, 0p; q_f the arm cell's size).  Each other
+// last one), with corners t.hidx(q, c), face geometry sg = t.slot_geom(f)
+// and transverse offsets t.hdt(q, k, sg) (k: m0, p0, 0m, 0p).  Each other
 // face f is regular: one corner t.id[f] carries its value and its transverse
 // products d_m0 d_p0, d_0m d_0p are 0, so it adds nothing to the m_ij of
 // correct_naive_second_derivatives nor to the quadratic corrections.  The
@@ -369,35 +371,34 @@ __device__ __forceinline__ void correct_naive(const TB &t, int64_t n, R &Dxx, R 
 // those of the per-face functions above, in FP64 bit for bit.
 
 // interpolate_linearly on hanging face q (my_p4est_quad_neighbor_nodes_of_node.h:222-306).
-// The transverse sums are the arm cell's size q_f (cells are cubes in the
-// tiled encoding), so on a dyadic geometry the division is a multiplication
-// by the exact inverse iqf (squared in 3D).
-template <int D, class R, class G, class TB>
-__device__ __forceinline__ R hang_face_linear(const TB &t, int q, R qf, R iqf, const G *__restrict__ g) {
-  const R d_m0 = t.hdt(q, 0, qf), d_p0 = t.hdt(q, 1, qf);
+// The transverse sums are the arm cell's transverse extents (sg.qt1, qt2), so on a
+// dyadic geometry the division is a multiplication by their exact inverses.
+template <int D, class R, class G, class TB, class SG>
+__device__ __forceinline__ R hang_face_linear(const TB &t, int q, const SG &sg, const G *__restrict__ g) {
+  const R d_m0 = t.hdt(q, 0, sg), d_p0 = t.hdt(q, 1, sg);
   if constexpr (D == 2) {
     const R num = qfma(R(g[t.hidx(q, 0)]), d_p0, R(g[t.hidx(q, 1)]) * d_m0);
-    if (t.pow2) return num * iqf;
+    if (t.pow2) return num * sg.iqt1;
     return qdiv(num, d_m0 + d_p0);
   } else {
-    const R d_0m = t.hdt(q, 2, qf), d_0p = t.hdt(q, 3, qf);
+    const R d_0m = t.hdt(q, 2, sg), d_0p = t.hdt(q, 3, sg);
     const R w_mm = d_p0 * d_0p, w_pm = d_m0 * d_0p, w_mp = d_p0 * d_0m, w_pp = d_m0 * d_0m;
     const R num = qfma(R(g[t.hidx(q, 0)]), w_mm,
                        qfma(R(g[t.hidx(q, 2)]), w_mp, qfma(R(g[t.hidx(q, 1)]), w_pm, R(g[t.hidx(q, 3)]) * w_pp)));
-    if (t.pow2) return num * (iqf * iqf);
+    if (t.pow2) return num * (sg.iqt1 * sg.iqt2);
     return qdiv(num, (d_m0 + d_p0) * (d_0m + d_0p));
   }
 }
 
 // the quadratic correction of hanging face q (axis a), as quadratic_corrections
-template <int D, class R, class TB>
-__device__ __forceinline__ R hang_correction(const TB &t, int q, int a, R qf, const R *dd) {
+template <int D, class R, class TB, class SG>
+__device__ __forceinline__ R hang_correction(const TB &t, int q, int a, const SG &sg, const R *dd) {
   const R h = R(0.5);
   if constexpr (D == 2) {
-    return h * t.hdt(q, 0, qf) * t.hdt(q, 1, qf) * (a == 0 ? dd[1] : dd[0]);
+    return h * t.hdt(q, 0, sg) * t.hdt(q, 1, sg) * (a == 0 ? dd[1] : dd[0]);
   } else {
     const R dd1 = (a == 0) ? dd[1] : dd[0], dd2 = (a == 2) ? dd[1] : dd[2];
-    return qfma(h * t.hdt(q, 0, qf) * t.hdt(q, 1, qf), dd1, h * t.hdt(q, 2, qf) * t.hdt(q, 3, qf) * dd2);
+    return qfma(h * t.hdt(q, 0, sg) * t.hdt(q, 1, sg), dd1, h * t.hdt(q, 2, sg) * t.hdt(q, 3, sg) * dd2);
   }
 }
 
@@ -414,8 +415,8 @@ __device__ __forceinline__ void hang_face_values(const TB &t, const G *__restric
     const int f = t.hface(q);
     if (q == 0 || f < F) {
       const auto sg = t.slot_geom(f);
-      R v = hang_face_linear<D, R>(t, q, sg.qf, sg.iqf, g);
-      if constexpr (CORR) v -= hang_correction<D, R>(t, q, f >> 1, sg.qf, dd);
+      R v = hang_face_linear<D, R>(t, q, sg, g);
+      if constexpr (CORR) v -= hang_correction<D, R>(t, q, f >> 1, sg, dd);
       vq[q] = v;
 #pragma unroll
       for (int i = 0; i < F; ++i) fnb[i] = (i == f) ? v : fnb[i];
@@ -434,7 +435,7 @@ __device__ __forceinline__ void hang_face_values_dd(const TB &t, const G *__rest
     const int f = t.hface(q);
     if (q == 0 || f < F) {
       const auto sg = t.slot_geom(f);
-      const R v = hang_face_linear<D, R>(t, q, sg.qf, sg.iqf, dd + int64_t(f >> 1) * stride);
+      const R v = hang_face_linear<D, R>(t, q, sg, dd + int64_t(f >> 1) * stride);
 #pragma unroll
       for (int i = 0; i < F; ++i) out[i] = (i == f) ? v : out[i];
     }
@@ -454,7 +455,7 @@ __device__ __forceinline__ void hang_correct_naive(const TB &t, R &Dxx, R &Dyy, 
     if (q == 0 || f < F) {
       const int a = f >> 1;
       const auto sg = t.slot_geom(f);
-      const R qf = sg.qf, df = sg.df, iqf = sg.iqf, dp = sg.dp, dm = sg.dm, iqp = sg.iqp;
+      const R df = sg.df, iqf = sg.iqf, dp = sg.dp, dm = sg.dm, iqp = sg.iqp;
       auto term = [&](R x) {
         // div_d(x, f), then div_dsum(., a)
         if constexpr (irregular_fastdiv<R, TB>()) {
@@ -466,11 +467,11 @@ __device__ __forceinline__ void hang_correct_naive(const TB &t, R &Dxx, R &Dyy, 
           return qdiv(qdiv(x, df), dp + dm);
         }
       };
-      const R T1 = term(t.hdt(q, 0, qf) * t.hdt(q, 1, qf));
+      const R T1 = term(t.hdt(q, 0, sg) * t.hdt(q, 1, sg));
 #pragma unroll
       for (int b = 0; b < D; ++b) m[b][0] = m[b][0] + ((a == b) ? T1 : R(0));
       if constexpr (D == 3) {
-        const R T2 = term(t.hdt(q, 2, qf) * t.hdt(q, 3, qf));
+        const R T2 = term(t.hdt(q, 2, sg) * t.hdt(q, 3, sg));
 #pragma unroll
         for (int b = 0; b < D; ++b) m[b][1] = m[b][1] + ((a == b) ? T2 : R(0));
       }
@@ -545,7 +546,7 @@ __device__ __forceinline__ void ngbd_quadratic(const TB &t, int64_t n, const G *
     for (int q = 0; q < TB::NH; ++q) {
       const int f = t.hface(q);
       if (q == 0 || f < F) {
-        const R v = vq[q] - hang_correction<D, R>(t, q, f >> 1, t.slot_geom(f).qf, dd);
+        const R v = vq[q] - hang_correction<D, R>(t, q, f >> 1, t.slot_geom(f), dd);
 #pragma unroll
         for (int i = 0; i < F; ++i) fnb[i] = (i == f) ? v : fnb[i];
       }

@@ -87,12 +87,12 @@ struct Builder {
       std::lock_guard<std::mutex> g(mu);
       for (int a = 0; a < D; ++a) bmax[a] = std::max(bmax[a], loc[a]);
     });
-    // one scale for all axes: cells are cubes in logical space, so the
-    // transverse extent of an arm cell equals its extent along the face axis
-    // (boxes whose edges differ by non-power-of-two ratios fail verify()).
-    double b = 0.0;
-    for (int a = 0; a < D; ++a) b = std::max(b, bmax[a]);
-    for (int a = 0; a < D; ++a) base[a] = b;
+    // one scale per axis: a cell of level l spans base[a] * 2^-l along axis
+    // a, the same l on every axis (the box edges may differ, cells need not
+    // be cubes).  The largest arm cell along each axis is a cell of the
+    // coarsest level present; a grid where that fails on some axis gives
+    // inconsistent levels, which verify() rejects.
+    for (int a = 0; a < D; ++a) base[a] = bmax[a];
     std::atomic<int> lr{0};
     parallel_for(N, [&](int64_t b, int64_t e) {
       int m = 0;
@@ -346,6 +346,8 @@ struct Builder {
 
   void run(TileHost &H) {
     scan_scales();
+    if (!tile_scales_supported(base, D))
+      throw std::runtime_error("cells that are not cubes on a box whose edges are not dyadic");
     const int64_t nCand = (N + TileLimits<D>::TMAX - 1) / TileLimits<D>::TMAX;
     std::vector<std::vector<Tile>> parts(nCand);
     std::atomic<int64_t> cursor{0};

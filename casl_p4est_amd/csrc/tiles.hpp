@@ -33,6 +33,7 @@
 // verifies this for every record.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <vector>
 
@@ -161,6 +162,23 @@ static_assert(TileLimits<2>::LMAX <= (1 << UniRec<2>::IDX_BITS) && TileLimits<2>
               "2D tile limits exceed the uniform record's fields");
 static_assert(TileLimits<3>::LMAX <= (1 << UniRec<3>::IDX_BITS) && TileLimits<3>::DMAX <= (1 << UniRec<3>::SLOT_BITS),
               "3D tile limits exceed the uniform record's fields");
+
+// Every scale a normal power of two whose arm cells (down to 2^-15 of it) and
+// their inverses stay normal: the tiled kernels' exact shortcuts hold.
+inline bool dyadic_scales(const double *base, int dim) {
+  for (int a = 0; a < dim; ++a) {
+    int e = 0;
+    if (std::frexp(base[a], &e) != 0.5 || e < -90 || e > 90) return false;
+  }
+  return true;
+}
+// Scales the tiled kernels take: dyadic, or one scale on every axis (cubic
+// cells, whose hanging arms' transverse extents are the arm cell's size).
+inline bool tile_scales_supported(const double *base, int dim) {
+  bool cubic = true;
+  for (int a = 1; a < dim; ++a) cubic = cubic && base[a] == base[0];
+  return cubic || dyadic_scales(base, dim);
+}
 
 struct TileHost {
   int dim = 0;

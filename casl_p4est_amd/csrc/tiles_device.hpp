@@ -16,6 +16,12 @@ struct TileGeom {
   R unit[3];      // finest cell size per axis
   R inv_base[3];  // 1 / base, exact when pow2
   int pow2;       // every base[a] is a power of two: all arm lengths are, too
+  int cubic;      // one scale on every axis (cubic cells)
+  // pow2 only: the exponents e_a of base[a] = 2^e_a, each + 128 in 8 bits
+  // (axis a at bits 8a), and the finest level (unit[a] = 2^(e_a - lmax));
+  // a run-time axis picks its exponent with a shift
+  uint32_t exp_pack;
+  int lmax;
   // pow2 only: the bit pattern (FP32) or high word (FP64) of base and 1/base,
   // whose exponent field a level nibble moves down or up
   uint32_t base_bits[3], inv_bits[3];
@@ -25,6 +31,9 @@ __device__ __forceinline__ double pow2neg(double, int l) { return __longlong_as_
 __device__ __forceinline__ float pow2neg(float, int l) { return __int_as_float((127 - l) << 23); }
 __device__ __forceinline__ double pow2pos(double, int l) { return __longlong_as_double((1023ll + l) << 52); }
 __device__ __forceinline__ float pow2pos(float, int l) { return __int_as_float((127 + l) << 23); }
+// 2^e for a normal-range exponent e
+__device__ __forceinline__ double pow2i(double, int e) { return __longlong_as_double((1023ll + e) << 52); }
+__device__ __forceinline__ float pow2i(float, int e) { return __int_as_float((127 + e) << 23); }
 
 // Face distances of a record: arm-cell sizes from the level nibbles, wall arms
 // -(d_opposite) - (second-degree cell size) (construct_neighbors :609-719).
@@ -106,7 +115,7 @@ struct RegStencil : DecodedDist<D, R> {
 
 // Stencil of a uniform node on a dyadic geometry (every arm cell a power of
 // two; the kernels take this accessor only when TileGeom::pow2): one corner
-// per face, all arms q = base * 2^-level long, no wall.  The per-face arrays
+// per face, the arms along axis a q_a = base_a * 2^-level long, no wall.  The per-face arrays
 // hold the same value so the shared stencil code reads them as it reads a
 // RegStencil's.
 template <int D, class R>
@@ -119,7 +128,7 @@ struct UniStencil {
   static constexpr int F = 2 * D;
   uint16_t id[F];
   R dist[F], q[F], iq[F];
-  R iq2;  // 1 / q^2
+  R iq2[D];  // 1 / q_a^2 per axis
   __device__ __forceinline__ int32_t idx(int f, int, int64_t) const { return id[f]; }
   __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
   __device__ __forceinline__ R dt(int f, int k, int64_t) const { return (k & 1) ? q[f] : R(0); }
@@ -127,14 +136,14 @@ struct UniStencil {
   __device__ __forceinline__ int load(const uint32_t *w, const TileGeom<R> &g) {
     int slot, level;
     uni_unpack<D>(w, id, slot, level);
-    const R qq = g.base[0] * pow2neg(R(0), level), iqq = g.inv_base[0] * pow2pos(R(0), level);
 #pragma unroll
-    for (int f = 0; f < F; ++f) {
-      dist[f] = qq;
-      q[f] = qq;
-      iq[f] = iqq;
+    for (int a = 0; a < D; ++a) {
+      const R qq = g.base[a] * pow2neg(R(0), level), iqq = g.inv_base[a] * pow2pos(R(0), level);
+      dist[2 * a] = dist[2 * a + 1] = qq;
+      q[2 * a] = q[2 * a + 1] = qq;
+      iq[2 * a] = iq[2 * a + 1] = iqq;
+      iq2[a] = iqq * iqq;
     }
-    iq2 = iqq * iqq;
     return slot;
   }
 };
@@ -165,19 +174,27 @@ struct HangStencil : DecodedDist<D, R> {
   static constexpr int NW = WORDS / 2;  // 32-bit words (the record is whole 8-byte words)
   uint16_t id[F];
   uint32_t e[NW];
-  R unit;
-  bool pow2;
+  bool pow2, cubic;
   uint32_t geo;
   R base[D], inv_base[D];  // the geometry's (uniform values)
+  R unit0;                 // finest cell (of every axis when cubic)
+  uint32_t exp_pack;       // TileGeom::exp_pack and lmax (pow2)
+  int lmax;
   // The arm cell size, its inverse and the face distances of the axis of a
   // face f known only at run time, from the level nibbles and wall bits as
   // decode_dist computes them (no indexing of the per-face arrays, which
-  // would move them to scratch memory).
+  // would move them to scratch memory), and the arm cell's extents along the
+  // two transverse axes (x -> y, z; y -> x, z; z -> x, y): the arm cell's
+  // own size when the cells are cubes, else (the tiled encoding then needs a
+  // pow2 geometry) from the packed exponents.
   struct SlotGeom {
-    R qf, iqf;  // arm cell of face f and its inverse
-    R df;       // face distance of f
-    R dp, dm;   // face distances of the + and - faces of f's axis
-    R iqp;      // inverse arm cell of the + face
+    R qf, iqf;        // arm cell of face f and its inverse
+    R df;             // face distance of f
+    R dp, dm;         // face distances of the + and - faces of f's axis
+    R iqp;            // inverse arm cell of the + face
+    R qt1, qt2;       // the arm cell's transverse extents
+    R iqt1, iqt2;     // and their inverses
+    R ut1, ut2;       // the finest cell along the transverse axes (offset unit)
   };
   __device__ __forceinline__ SlotGeom slot_geom(int f) const {
     const int a = f >> 1;
@@ -192,12 +209,28 @@ struct HangStencil : DecodedDist<D, R> {
     const R de = ((geo >> (24 + 2 * a)) & 1u) ? -qo - qe : qe;
     const R dO = ((geo >> (25 + 2 * a)) & 1u) ? -de - qo : qo;
     SlotGeom r;
+    const int l = (f & 1) ? lo : le;
     r.qf = (f & 1) ? qo : qe;
-    r.iqf = ib * pow2pos(R(0), (f & 1) ? lo : le);
+    r.iqf = ib * pow2pos(R(0), l);
     r.df = (f & 1) ? dO : de;
     r.dp = dO;
     r.dm = de;
     r.iqp = ib * pow2pos(R(0), lo);
+    if (!cubic) {
+      // first transverse axis: y for x, else x; second: y for z, else z
+      const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+      const int e1 = int((exp_pack >> (8 * t1)) & 255u) - 128, e2 = int((exp_pack >> (8 * t2)) & 255u) - 128;
+      r.qt1 = pow2i(R(0), e1 - l);
+      r.iqt1 = pow2i(R(0), l - e1);
+      r.ut1 = pow2i(R(0), e1 - lmax);
+      r.qt2 = pow2i(R(0), e2 - l);
+      r.iqt2 = pow2i(R(0), l - e2);
+      r.ut2 = pow2i(R(0), e2 - lmax);
+    } else {
+      r.qt1 = r.qt2 = r.qf;
+      r.iqt1 = r.iqt2 = r.iqf;
+      r.ut1 = r.ut2 = unit0;
+    }
     return r;
   }
   __device__ __forceinline__ uint32_t word(int i) const { return (e[i >> 1] >> (16 * (i & 1))) & 0xffffu; }
@@ -206,10 +239,10 @@ struct HangStencil : DecodedDist<D, R> {
     return D == 2 ? int(word(0)) : int((word(0) >> (4 * q)) & 15u);
   }
   __device__ __forceinline__ int32_t hidx(int q, int c) const { return int32_t(word(1 + q * SW + c)); }
-  // transverse offset k (m0, p0, 0m, 0p) of slot q, qf its arm cell's size
-  __device__ __forceinline__ R hdt(int q, int k, R qf) const {
-    const R o = R(word(1 + q * SW + C + (k >> 1))) * unit;
-    return (k & 1) ? qf - o : o;
+  // transverse offset k (m0, p0, 0m, 0p) of slot q; sg: the slot's face geometry
+  __device__ __forceinline__ R hdt(int q, int k, const SlotGeom &sg) const {
+    const R o = R(word(1 + q * SW + C + (k >> 1))) * ((k >> 1) ? sg.ut2 : sg.ut1);
+    return (k & 1) ? ((k >> 1) ? sg.qt2 : sg.qt1) - o : o;
   }
   __device__ __forceinline__ int32_t idx(int f, int c, int64_t) const {
     int32_t r = id[f];
@@ -221,7 +254,19 @@ struct HangStencil : DecodedDist<D, R> {
   __device__ __forceinline__ R dt(int f, int k, int64_t) const {
     R r = (k & 1) ? this->q[f] : R(0);
 #pragma unroll
-    for (int q = 0; q < NH; ++q) r = (hface(q) == f) ? hdt(q, k, this->q[f]) : r;
+    for (int q = 0; q < NH; ++q) r = (hface(q) == f) ? hdt(q, k, slot_geom(f)) : r;
+    return r;
+  }
+  // 1 / (the sum of face f's transverse offsets, times the second sum in 3D):
+  // the interpolation's normalisation (pow2 geometry)
+  __device__ __forceinline__ R inv_norm(int f) const {
+    R r = D == 2 ? this->iq[f] : this->iq[f] * this->iq[f];
+#pragma unroll
+    for (int q = 0; q < NH; ++q)
+      if (hface(q) == f) {
+        const SlotGeom sg = slot_geom(f);
+        r = D == 2 ? sg.iqt1 : sg.iqt1 * sg.iqt2;
+      }
     return r;
   }
   // rec: the node's record; xrec: its extension record (WORDS uint16)
@@ -235,14 +280,17 @@ struct HangStencil : DecodedDist<D, R> {
       e[2 * i] = v.x;
       e[2 * i + 1] = v.y;
     }
-    unit = g.unit[0];
     pow2 = g.pow2 != 0;
+    cubic = g.cubic != 0;
     geo = rec.geo;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       base[k] = g.base[k];
       inv_base[k] = g.inv_base[k];
     }
+    unit0 = g.unit[0];
+    exp_pack = g.exp_pack;
+    lmax = g.lmax;
     this->decode(rec.geo, g);
   }
 };

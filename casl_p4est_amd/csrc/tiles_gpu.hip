@@ -83,9 +83,10 @@ struct DTab {
   }
 };
 
-// The scales of the encoding.
+// The scales of the encoding: per axis, the coarsest arm cell and the finest
+// cell (a cell of level l spans base[a] * 2^-l along axis a).
 struct Scales {
-  double base, unit;
+  double base[3], unit[3];
   int lmax;
 };
 
@@ -119,18 +120,21 @@ __global__ void k_check_convert(DTab<D> t, float *__restrict__ dist, int *__rest
 }
 
 // ---------------------------------------------------------------- scales
-// max arm cell (non-negative doubles order like their bit patterns)
+// max arm cell per axis, out[D] (non-negative doubles order like their bit
+// patterns)
 template <int D>
 __global__ void k_arm_max(DTab<D> t, unsigned long long *__restrict__ out, int *__restrict__ bad) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
-  double m = 0.0;
-  for (int f = 0; f < DTab<D>::F; ++f) {
-    const double q = t.arm_cell(f, n);
-    if (!(q > 0)) flag(bad, BAD_SCALE);
-    m = fmax(m, q);
+  for (int a = 0; a < D; ++a) {
+    double m = 0.0;
+    for (int f = 2 * a; f < 2 * a + 2; ++f) {
+      const double q = t.arm_cell(f, n);
+      if (!(q > 0)) flag(bad, BAD_SCALE);
+      m = fmax(m, q);
+    }
+    atomicMax(out + a, (unsigned long long)__double_as_longlong(m));
   }
-  atomicMax(out, (unsigned long long)__double_as_longlong(m));
 }
 
 // level_of of every arm cell (as the host builder: nearest integer of
@@ -149,11 +153,11 @@ __device__ __forceinline__ int level_of(double base, double q, int *bad) {
 }
 
 template <int D>
-__global__ void k_level_max(DTab<D> t, double base, int *__restrict__ out, int *__restrict__ bad) {
+__global__ void k_level_max(DTab<D> t, Scales sc, int *__restrict__ out, int *__restrict__ bad) {
   const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (n >= t.N) return;
   int m = 0;
-  for (int f = 0; f < DTab<D>::F; ++f) m = max(m, level_of(base, t.arm_cell(f, n), bad));
+  for (int f = 0; f < DTab<D>::F; ++f) m = max(m, level_of(sc.base[f >> 1], t.arm_cell(f, n), bad));
   atomicMax(out, m);
 }
 
@@ -324,7 +328,7 @@ __device__ int encode(const DTab<D> &t, const TileView &v, int tl, int64_t n, Ti
   uint32_t geo = 0;
   int hf[2] = {HANG_NONE, HANG_NONE}, k = 0;
   for (int f = 0; f < F; ++f) {
-    geo |= uint32_t(level_of(v.sc.base, t.arm_cell(f, n), bad)) << (4 * f);
+    geo |= uint32_t(level_of(v.sc.base[f >> 1], t.arm_cell(f, n), bad)) << (4 * f);
     if (t.w(f, n)) geo |= 1u << (24 + f);
     double w[4];
     t.weights(f, n, w);
@@ -375,17 +379,17 @@ __device__ int encode(const DTab<D> &t, const TileView &v, int tl, int64_t n, Ti
         }
         fe[c] = uint16_t(l);
       }
-      // units(): the offset in finest cells
+      // units(): the offset in finest cells of its transverse axis
       for (int m = 0; m < (D == 3 ? 2 : 1); ++m) {
         const double val = t.dt(f, 2 * m, n);
-        const double kk = val / v.sc.unit;
-        if (!(kk >= 0) || kk > 65535 || floor(kk) != kk || kk * v.sc.unit != val) {
+        const double u = v.sc.unit[m == 0 ? t1 : t2];
+        const double kk = val / u;
+        if (!(kk >= 0) || kk > 65535 || floor(kk) != kk || kk * u != val) {
           flag(bad, BAD_UNITS);
           return -1;
         }
         fe[C + m] = uint16_t(kk);
       }
-      (void)t2;
     }
   }
   r.geo = geo;
@@ -398,11 +402,11 @@ template <int D>
 __device__ void verify(const DTab<D> &t, const TileView &v, int tl, int64_t n, const TileRec &r, const uint16_t *e,
                        int *bad) {
   constexpr int F = DTab<D>::F, C = DTab<D>::C, SW = HangExt<D>::SW;
-  const double base = v.sc.base;
+  const double *base = v.sc.base;
   double dd[6];
-  for (int f = 0; f < F; ++f) dd[f] = base * ldexp(1.0, -int((r.geo >> (4 * f)) & 15u));
+  for (int f = 0; f < F; ++f) dd[f] = base[f >> 1] * ldexp(1.0, -int((r.geo >> (4 * f)) & 15u));
   for (int f = 0; f < F; ++f) {
-    if ((r.geo >> (24 + f)) & 1u) dd[f] = -dd[f ^ 1] - base * ldexp(1.0, -int((r.geo >> (4 * f)) & 15u));
+    if ((r.geo >> (24 + f)) & 1u) dd[f] = -dd[f ^ 1] - base[f >> 1] * ldexp(1.0, -int((r.geo >> (4 * f)) & 15u));
     if (dd[f] != t.df(f, n)) flag(bad, BAD_VERIFY);
   }
   int hf[2] = {HANG_NONE, HANG_NONE};
@@ -428,12 +432,17 @@ __device__ void verify(const DTab<D> &t, const TileView &v, int tl, int64_t n, c
     const uint16_t *fe = e + 1 + q * SW;
     for (int c = 0; c < C; ++c)
       if (w[c] != 0.0 && local_of(v, tl, t.nb(f, c, n)) != fe[c]) flag(bad, BAD_VERIFY);
-    const double qh = base * ldexp(1.0, -int((r.geo >> (4 * f)) & 15u));
-    const double m0 = fe[C] * v.sc.unit;
-    if (m0 != t.dt(f, 0, n) || qh - m0 != t.dt(f, 1, n)) flag(bad, BAD_VERIFY);
+    // the arm cell's transverse extents, at the face's level
+    int t1, t2;
+    transverse(D, f >> 1, t1, t2);
+    const double lv = ldexp(1.0, -int((r.geo >> (4 * f)) & 15u));
+    const double qh1 = base[t1] * lv;
+    const double m0 = fe[C] * v.sc.unit[t1];
+    if (m0 != t.dt(f, 0, n) || qh1 - m0 != t.dt(f, 1, n)) flag(bad, BAD_VERIFY);
     if (D == 3) {
-      const double zm = fe[C + 1] * v.sc.unit;
-      if (zm != t.dt(f, 2, n) || qh - zm != t.dt(f, 3, n)) flag(bad, BAD_VERIFY);
+      const double qh2 = base[t2] * lv;
+      const double zm = fe[C + 1] * v.sc.unit[t2];
+      if (zm != t.dt(f, 2, n) || qh2 - zm != t.dt(f, 3, n)) flag(bad, BAD_VERIFY);
     }
   }
 }
@@ -699,19 +708,25 @@ void build(const DTab<D> &tab, casl_stencil *st, hipStream_t s) {
   // scales
   DevBuf<unsigned long long> amax;
   DevBuf<int> lmx;
-  amax.ensure(1);
+  amax.ensure(D);
   lmx.ensure(1);
-  HIP_CHECK(hipMemsetAsync(amax.p, 0, sizeof(unsigned long long), s));
+  HIP_CHECK(hipMemsetAsync(amax.p, 0, D * sizeof(unsigned long long), s));
   HIP_CHECK(hipMemsetAsync(lmx.p, 0, sizeof(int), s));
   hipLaunchKernelGGL(k_arm_max<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, tab, amax.p, bad.p);
   HIP_CHECK(hipGetLastError());
-  const unsigned long long bits = read_back(amax.p, s);
+  // as the host builder: one scale per axis (axes past D keep the box edge 1)
+  unsigned long long bits[3] = {0, 0, 0};
+  HIP_CHECK(hipMemcpyAsync(bits, amax.p, D * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+  HIP_CHECK(hipStreamSynchronize(s));
   Scales sc;
-  std::memcpy(&sc.base, &bits, sizeof(double));
-  hipLaunchKernelGGL(k_level_max<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, tab, sc.base, lmx.p, bad.p);
+  for (int a = 0; a < 3; ++a) sc.base[a] = 1.0;
+  for (int a = 0; a < D; ++a) std::memcpy(&sc.base[a], &bits[a], sizeof(double));
+  hipLaunchKernelGGL(k_level_max<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, tab, sc, lmx.p, bad.p);
   HIP_CHECK(hipGetLastError());
   sc.lmax = read_back(lmx.p, s);
-  sc.unit = sc.base * std::ldexp(1.0, -sc.lmax);
+  for (int a = 0; a < 3; ++a) sc.unit[a] = sc.base[a] * std::ldexp(1.0, -sc.lmax);
+  if (!tile_scales_supported(sc.base, D))
+    throw std::runtime_error("cells that are not cubes on a box whose edges are not dyadic");
   int b = read_back(bad.p, s);
   if (b) throw std::runtime_error(bad_reason(b));
 
@@ -876,8 +891,8 @@ void build(const DTab<D> &tab, casl_stencil *st, hipStream_t s) {
   st->n_uniform = n_uni;
   st->n_irregular = n_ext;
   for (int a = 0; a < 3; ++a) {
-    st->geom_base[a] = sc.base;
-    st->geom_unit[a] = sc.unit;
+    st->geom_base[a] = sc.base[a];
+    st->geom_unit[a] = sc.unit[a];
   }
   st->tiled = true;
   // the independent bounds check before any kernel reads the encoding

@@ -1,8 +1,9 @@
 """Host-side grid objects: the role p4est, my_p4est_nodes_new and
 my_p4est_node_neighbors_t play in the reference.
 
-    Forest            one non-periodic tree on a box (my_p4est_brick_new with
-                      one tree + my_p4est_new + my_p4est_refine)
+    Forest            a brick of trees on a box, periodic or not per axis
+                      (my_p4est_brick_new, src/my_p4est_tools.c:37-170, +
+                      my_p4est_new + my_p4est_refine)
     Forest.refine_levelset
                       refine_levelset_cf applied max_lvl times
                       (src/my_p4est_refine_coarsen.cpp:17-89,
@@ -25,17 +26,27 @@ P4EST_MAXLEVEL = {2: 30, 3: 19}
 
 
 class Forest:
-    """A single-tree quadtree (dim=2) or octree (dim=3) on [box_min, box_max]."""
+    """A forest of quadtrees (dim=2) or octrees (dim=3) on [box_min, box_max]:
+    n_trees[a] trees along axis a (default one tree), periodic[a] wrapping
+    axis a (default none), as my_p4est_brick_new(n_xyz, xyz_min, xyz_max,
+    &brick, periodic)."""
 
-    def __init__(self, dim, box_min=None, box_max=None):
+    def __init__(self, dim, box_min=None, box_max=None, n_trees=None, periodic=None):
         if dim not in (2, 3):
             raise ValueError("dim must be 2 or 3")
         self.dim = dim
         self.box_min = np.asarray(box_min if box_min is not None else [0.0] * dim, dtype=np.float64)
         self.box_max = np.asarray(box_max if box_max is not None else [1.0] * dim, dtype=np.float64)
+        self.n_trees = np.asarray(n_trees if n_trees is not None else [1] * dim, dtype=np.int32)
+        self.periodic = np.asarray([int(bool(p)) for p in periodic] if periodic is not None else [0] * dim,
+                                   dtype=np.int32)
+        for v in (self.box_min, self.box_max, self.n_trees, self.periodic):
+            if v.shape != (dim,):
+                raise ValueError(f"box, trees and periodicity need {dim} entries")
         h = ctypes.c_void_p()
-        L.call("casl_forest_new", dim, self.box_min.ctypes.data_as(L.c_double_p),
-               self.box_max.ctypes.data_as(L.c_double_p), ctypes.byref(h))
+        L.call("casl_forest_new_brick", dim, self.box_min.ctypes.data_as(L.c_double_p),
+               self.box_max.ctypes.data_as(L.c_double_p), self.n_trees.ctypes.data_as(L.c_int32_p),
+               self.periodic.ctypes.data_as(L.c_int32_p), ctypes.byref(h))
         L.own(self, h, "casl_forest_destroy")
         self._nodes_built = False
 
@@ -83,8 +94,21 @@ class Forest:
                lvl.ctypes.data_as(L.c_int8_p), local.ctypes.data_as(L.c_int32_p) if local is not None else None)
         return xyz, lvl, local
 
+    def trees(self):
+        """(brick position of every tree [T][dim], tree of every leaf [L],
+        canonical tree of every node [N] or None before build_nodes)."""
+        nt = ctypes.c_int32()
+        L.call("casl_forest_trees", self._h, ctypes.byref(nt), None, None, None)
+        pos = np.empty((nt.value, 3), dtype=np.int32)
+        leaf = np.empty(self.num_leaves, dtype=np.int32)
+        node = np.empty(self.num_nodes, dtype=np.int32) if self._nodes_built else None
+        L.call("casl_forest_trees", self._h, None, pos.ctypes.data_as(L.c_int32_p), leaf.ctypes.data_as(L.c_int32_p),
+               node.ctypes.data_as(L.c_int32_p) if node is not None else None)
+        return pos[:, :self.dim], leaf, node
+
     def nodes(self):
-        """(logical unclamped coords [N][dim] int32, physical coords [N][dim])."""
+        """(logical unclamped coords in the node's tree [N][dim] int32,
+        physical coords [N][dim])."""
         n = self.num_nodes
         ixyz = np.empty((n, self.dim), dtype=np.int32)
         xyz = np.empty((n, self.dim), dtype=np.float64)
@@ -101,7 +125,7 @@ class Forest:
     def dxyz_min(self):
         """Smallest cell sizes per axis (get_dxyz_min)."""
         ml = self.max_level
-        return (self.box_max - self.box_min) / float(2 ** ml)
+        return (self.box_max - self.box_min) / self.n_trees / float(2 ** ml)
 
 
 class DeviceGrid:

@@ -42,7 +42,9 @@ extern "C" {
 #define CASL_ERR_NO_DEVICE         -2 /* no HIP device / library not usable */
 #define CASL_ERR_HIP               -3 /* a HIP runtime call failed          */
 #define CASL_ERR_OUT_OF_MEMORY     -4
-#define CASL_ERR_INEXACT_DISTANCES -5 /* stencil distances not dyadic       */
+#define CASL_ERR_INEXACT_DISTANCES -5 /* distances not exact in FP32 where
+                                         FP32 is required (batches, grids
+                                         built on the device)            */
 #define CASL_ERR_INVALID_STENCIL   -6 /* index out of range, bad wall arm   */
 #define CASL_ERR_NOT_BUILT         -7 /* forest step called out of order    */
 
@@ -84,9 +86,10 @@ typedef struct casl_flat_stencil {
 typedef struct casl_stencil casl_stencil_t;
 
 /* Validates the host tables and uploads them to the current device
- * (replaces init_neighbors' cached qnnn vector).  All distances must be
- * exactly representable in FP32 (true for every dyadic tree of power-of-two
- * size), otherwise CASL_ERR_INEXACT_DISTANCES. */
+ * (replaces init_neighbors' cached qnnn vector).  Distances are kept in FP32
+ * (exact for trees of dyadic size); when FP32 does not hold them exactly (a
+ * box such as [0, 2 pi]) an FP64 copy is kept too, and FP64 calls run the
+ * grid-wide kernels on it (mixed calls use the rounded FP32 copy). */
 int casl_stencil_create(const casl_flat_stencil *tables, casl_stencil_t **out);
 int casl_stencil_destroy(casl_stencil_t *st);
 int casl_stencil_info(const casl_stencil_t *st, int32_t *dim, int64_t *n_nodes);
@@ -384,7 +387,10 @@ int casl_bf16_gemm(int M, int N, int K, const uint16_t *A, int lda,
                    int ldc, void *stream);
 
 /* ------------------------------------------------ host forest (p4est role)
- * A single tree on the box [box_min, box_max] (one brick tree, non periodic). */
+ * A brick of n_trees[0] x n_trees[1] (x n_trees[2]) trees on the box
+ * [box_min, box_max], periodic or not per axis (my_p4est_brick_new,
+ * src/my_p4est_tools.c:37-170).  casl_forest_new is the brick of one
+ * non-periodic tree. */
 typedef struct casl_forest casl_forest_t;
 
 /* Level-set descriptions for refinement and sampling. */
@@ -415,20 +421,34 @@ int casl_levelset_eval(const casl_levelset *ls, int dim, int64_t n,
 
 int casl_forest_new(int dim, const double *box_min, const double *box_max,
                     casl_forest_t **out);
+/* n_trees: trees per axis (1..1024, NULL: 1), periodic: 0/1 per axis (NULL:
+ * none).  Trees are numbered along a Morton curve of their brick positions. */
+int casl_forest_new_brick(int dim, const double *box_min, const double *box_max,
+                          const int32_t *n_trees, const int32_t *periodic,
+                          casl_forest_t **out);
+/* Number of trees, brick position of each tree [T][3], tree of each leaf [L]
+ * and of each node [N] (the node's canonical tree: the lowest-numbered tree
+ * that holds it); any output may be NULL. */
+int casl_forest_trees(const casl_forest_t *f, int32_t *n_trees,
+                      int32_t *tree_pos, int32_t *leaf_tree,
+                      int32_t *node_tree);
 int casl_forest_destroy(casl_forest_t *f);
 /* refine_levelset_cf with splitting_criteria_cf_t(min_lvl, max_lvl, phi, lip,
  * band), applied max_lvl times (non-recursive refinement, no 2:1 balance). */
 int casl_forest_refine_levelset(casl_forest_t *f, int min_lvl, int max_lvl,
                                 double lip, double band,
                                 const casl_levelset *phi);
-/* my_p4est_nodes_new: canonical clamped corner nodes in Morton order. */
+/* my_p4est_nodes_new: canonical clamped corner nodes, by tree then in Morton
+ * order. */
 int casl_forest_build_nodes(casl_forest_t *f);
 int casl_forest_sizes(const casl_forest_t *f, int64_t *n_leaves,
                       int64_t *n_nodes, int32_t *max_level);
-/* leaves: xyz [L][dim] logical coords, level [L], local_nodes [L][2^dim] */
+/* leaves: xyz [L][dim] logical coords in their tree, level [L],
+ * local_nodes [L][2^dim] */
 int casl_forest_get_leaves(const casl_forest_t *f, int32_t *xyz,
                            int8_t *level, int32_t *local_nodes);
-/* nodes: logical (unclamped) coords [N][dim] and physical coords [N][dim] */
+/* nodes: logical (unclamped) coords in their canonical tree [N][dim] and
+ * physical coords [N][dim] */
 int casl_forest_get_nodes(const casl_forest_t *f, int32_t *ixyz,
                           double *xyz);
 /* sample_cf_on_nodes */

@@ -15,6 +15,8 @@ the smoke check and the benchmark's CPU-baseline leg, never by the product).
      - cell search        find_neighbor_cell_of_node src/my_p4est_hierarchy.cpp:569-601
                           (restated as a containment test over all leaves)
      - stencils           construct_neighbors     src/my_p4est_node_neighbors.cpp:114-1201
+     - bricks, periodicity my_p4est_brick_new     src/my_p4est_tools.c:37-170, and the
+                          periodic shifts of construct_neighbors (:345-357 and repeats)
    Pure Python loops: meant for trees of at most a few thousand nodes.
 """
 import ctypes
@@ -329,11 +331,198 @@ def brute_stencil(leaves_xyz, leaves_lvl, D, box_min=None, box_max=None):
     return nbr, d, dtr, wall, nodes
 
 
-def refine_decision(q_xyz, level, D, min_lvl, max_lvl, lip, band_factor, phi, box_min=None, box_max=None):
-    """refine_levelset_cf for one quadrant (src/my_p4est_refine_coarsen.cpp:17-89)."""
+def brick_geometry(D, box_min, box_max, n_trees):
+    """Tree vertex coordinates of my_p4est_brick_new (src/my_p4est_tools.c:
+    138-168): tree i spans [xyz_min + i dx, xyz_min + (i + 1) dx] with
+    dx = (xyz_max - xyz_min) / n.  Returns per-axis lists (vertex min,
+    extent = vertex max - vertex min) by brick position, and the domain
+    corners as the vertices give them."""
+    vmin, ext, dom = [], [], []
+    for a in range(D):
+        dx = (float(box_max[a]) - float(box_min[a])) / float(n_trees[a])
+        lo = [float(box_min[a]) + i * dx for i in range(n_trees[a])]
+        hi = [float(box_min[a]) + (i + 1) * dx for i in range(n_trees[a])]
+        vmin.append(lo)
+        ext.append([h - l for h, l in zip(hi, lo)])
+        dom.append((float(box_min[a]) + 0 * dx, float(box_min[a]) + n_trees[a] * dx))
+    return vmin, ext, dom
+
+
+def brute_stencil_brick(leaves_xyz, leaves_lvl, leaf_tree, tree_pos, D, box_min, box_max, n_trees, periodic):
+    """construct_neighbors on a brick of trees, periodic or not per axis
+    (src/my_p4est_node_neighbors.cpp:114-1201, the periodic shifts :345-357
+    and their repeats per face), by direct geometric search in brick
+    coordinates (tree position * ROOT_LEN + logical coordinate):
+
+      - a node's owner is found by listing every (tree, coordinate) that
+        holds it, periodic images included, and keeping the least by tree
+        number then p4est_quadrant_compare of the clamped coordinates
+        (p4est_node_canonicalize's rule);
+      - nodes are ordered by owner tree, then p4est_quadrant_compare;
+      - the cell of a perturbed point is the one leaf whose brick box holds
+        it (off a non-periodic boundary: none; across a periodic one the
+        point wraps).
+
+    Returns (nbr, d, d_tr, wall, nodes [(tree, unclamped coords)])."""
+    ML = MAXLEVEL[D]
+    R = 1 << ML
+    F, C, T = 2 * D, 1 << (D - 1), 2 * (D - 1)
+    leaves_xyz = [tuple(int(v) for v in q) for q in leaves_xyz]
+    leaves_lvl = [int(v) for v in leaves_lvl]
+    leaf_tree = [int(v) for v in leaf_tree]
+    tree_pos = [tuple(int(v) for v in p[:D]) for p in tree_pos]
+    tree_of = {p: t for t, p in enumerate(tree_pos)}
+    span = [n_trees[a] * R for a in range(D)]
+    vmin, ext, dom = brick_geometry(D, box_min, box_max, n_trees)
+    lens = [1 << (ML - lv) for lv in leaves_lvl]
+    gleaf = [tuple(tree_pos[t][a] * R + q[a] for a in range(D)) for q, t in zip(leaves_xyz, leaf_tree)]
+
+    def fold(g):
+        return tuple(g[a] % span[a] if periodic[a] else g[a] for a in range(D))
+
+    def owner(g):
+        per_axis = []
+        for a in range(D):
+            images = [g[a]] + ([g[a] + span[a]] if periodic[a] and g[a] == 0 else [])
+            cands = [(i, v - i * R) for v in images for i in range(n_trees[a]) if i * R <= v <= (i + 1) * R]
+            per_axis.append(cands)
+        best = None
+        for combo in __import__("itertools").product(*per_axis):
+            t = tree_of[tuple(c[0] for c in combo)]
+            loc = tuple(c[1] for c in combo)
+            clamped = tuple(min(v, R - 1) for v in loc)
+            if best is None or t < best[0] or (t == best[0] and quadrant_compare(clamped, best[2], D) < 0):
+                best = (t, loc, clamped)
+        return best
+
+    pts = {}
+    for gq, ln in zip(gleaf, lens):
+        for k in range(1 << D):
+            g = fold(tuple(gq[a] + ((k >> a) & 1) * ln for a in range(D)))
+            if g not in pts:
+                pts[g] = owner(g)
+    order = sorted(pts.items(), key=functools.cmp_to_key(
+        lambda x, y: (x[1][0] - y[1][0]) if x[1][0] != y[1][0] else quadrant_compare(x[1][2], y[1][2], D)))
+    index = {g: i for i, (g, _) in enumerate(order)}
+    nodes = [(own[0], own[1]) for _, own in order]
+    gnode = [g for g, _ in order]
+    N = len(order)
+
+    def corner_index(leaf, k):
+        return index[fold(tuple(gleaf[leaf][a] + ((k >> a) & 1) * lens[leaf] for a in range(D)))]
+
+    def cell_at(p):
+        p = list(p)
+        for a in range(D):
+            if periodic[a]:
+                p[a] %= span[a]
+            elif p[a] < 0 or p[a] > span[a]:
+                return -1
+        hits = [l for l in range(len(gleaf))
+                if all(gleaf[l][a] <= p[a] < gleaf[l][a] + lens[l] for a in range(D))]
+        assert len(hits) == 1, f"point {p} in {len(hits)} leaves"
+        return hits[0]
+
+    def coord(n, a):
+        t, loc = nodes[n]
+        i = tree_pos[t][a]
+        return ext[a][i] * (loc[a] / float(R)) + vmin[a][i]
+
+    def transverse(a):
+        return [1 - a] if D == 2 else [b for b in range(3) if b != a]
+
+    nbr = np.full((F, C, N), -1, dtype=np.int32)
+    d = np.zeros((F, N))
+    dtr = np.zeros((F, T, N))
+    wall = np.zeros(N, dtype=np.uint8)
+
+    def arm(leaf, a):
+        return ext[a][tree_pos[leaf_tree[leaf]][a]] * (lens[leaf] / float(R))
+
+    def fill(n, f, leaf, side):
+        a = f >> 1
+        tr = transverse(a)
+        for c in range(C):
+            k = side << a
+            k |= (c & 1) << tr[0]
+            if D == 3:
+                k |= ((c >> 1) & 1) << tr[1]
+            nbr[f, c, n] = corner_index(leaf, k)
+        qh = lens[leaf] / float(R)
+        i = tree_pos[leaf_tree[leaf]]
+        for t, ax in enumerate(tr):
+            e = ext[ax][i[ax]]
+            thr = e * 1.0 / float(R)
+            X = coord(n, ax)
+            QX = e * (leaves_xyz[leaf][ax] / float(R)) + vmin[ax][i[ax]]
+            m0 = (X - QX) if abs(X - QX) > thr else 0.0
+            if periodic[ax] and X < QX - e * qh / 4.0:
+                w = m0 + (dom[ax][1] - dom[ax][0])
+                m0 = w if abs(w) > thr else 0.0
+            p0 = (e * qh - m0) if abs(e * qh - m0) > thr else 0.0
+            dtr[f, 2 * t, n] = m0
+            dtr[f, 2 * t + 1, n] = p0
+
+    for n in range(N):
+        g = gnode[n]
+        cells = {}
+        for dirv in range(1 << D):
+            cells[dirv] = cell_at(tuple(g[a] + (1 if (dirv >> a) & 1 else -1) for a in range(D)))
+        lvl = {k: (-1 if v < 0 else leaves_lvl[v]) for k, v in cells.items()}
+        walls = []
+        for f in range(F):
+            a, s_ = f >> 1, f & 1
+            cands = [k for k in range(1 << D) if ((k >> a) & 1) == s_]
+            if all(cells[k] < 0 for k in cands):
+                walls.append(f)
+                continue
+            best = cands[0]
+            for k in cands[1:]:
+                if lvl[best] < lvl[k]:
+                    best = k
+            leaf = cells[best]
+            d[f, n] = arm(leaf, a)
+            fill(n, f, leaf, s_)
+        for f in walls:
+            a, s_ = f >> 1, f & 1
+            so = 1 - s_
+            cands = [k for k in range(1 << D) if ((k >> a) & 1) == so]
+            best, lbest = None, -1
+            for k in cands:
+                if lbest < lvl[k]:
+                    best, lbest = k, lvl[k]
+            qmin = cells[best]
+            corner = so << a
+            for b in range(D):
+                if b != a:
+                    corner |= (0 if (best >> b) & 1 else 1) << b
+            pt = gnode[corner_index(qmin, corner)]
+            qtmp = cell_at(tuple(pt[b] + (1 if (best >> b) & 1 else -1) for b in range(D)))
+            assert qtmp >= 0
+            d[f, n] = -d[f ^ 1, n] - arm(qtmp, a)
+            fill(n, f, qtmp, so)
+        w = 0
+        for a in range(D):
+            if periodic[a]:
+                continue
+            if g[a] == 0:
+                w |= 1 << (2 * a)
+            if g[a] == span[a]:
+                w |= 1 << (2 * a + 1)
+        wall[n] = w
+    return nbr, d, dtr, wall, nodes
+
+
+def refine_decision(q_xyz, level, D, min_lvl, max_lvl, lip, band_factor, phi, box_min=None, box_max=None,
+                    tree_min=None, tree_ext=None):
+    """refine_levelset_cf for one quadrant (src/my_p4est_refine_coarsen.cpp:17-89);
+    tree_min / tree_ext: the quadrant's tree vertex minimum and extent (a
+    brick), else the box."""
     bmin = [0.0] * D if box_min is None else list(box_min)
     bmax = [1.0] * D if box_max is None else list(box_max)
     size = [bmax[a] - bmin[a] for a in range(D)]
+    if tree_min is not None:
+        bmin, size = list(tree_min), list(tree_ext)
     ML = MAXLEVEL[D]
     R = float(1 << ML)
     if level < min_lvl:

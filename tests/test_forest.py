@@ -107,3 +107,91 @@ def test_invalid_arguments_raise():
         f.refine_levelset(ls.Spheres([[0.5, 0.5, 0.5]], [0.1]), 1, 3)  # 3D shape on a 2D forest
     with pytest.raises(C.CaslError):
         C.flat_stencil_from_forest(f)  # nodes not built
+
+
+BRICKS = [
+    # dim, box, trees per axis, periodic, shape, lmin, lmax
+    (2, ([0.0, 0.0], [2.0, 1.0]), [2, 1], [0, 0], ls.Sphere((0.9, 0.45), 0.3), 1, 5),
+    (2, ([0.0, 0.0], [1.0, 1.0]), [1, 1], [1, 1], ls.Sphere((0.1, 0.5), 0.25), 2, 5),
+    (2, ([-1.0, 0.0], [2.0, 2.0]), [3, 2], [1, 0], ls.Star(center=(0.0, 0.9), r0=0.5, amplitude=0.1, lobes=5), 1, 4),
+    (3, ([0.0, 0.0, 0.0], [2.0, 1.0, 1.0]), [2, 1, 1], [0, 1, 0], ls.Sphere((0.95, 0.1, 0.5), 0.3), 1, 3),
+    (3, ([0.0, 0.0, 0.0], [1.0, 1.0, 1.0]), [1, 2, 2], [1, 1, 1], ls.Sphere((0.05, 0.5, 0.95), 0.3), 1, 3),
+]
+
+
+def build_brick(dim, box, n_trees, periodic, shape, lmin, lmax, lip=2.4):
+    f = C.Forest(dim, box[0], box[1], n_trees=n_trees, periodic=periodic)
+    f.refine_levelset(shape, lmin, lmax, lip=lip).build_nodes()
+    return f, C.flat_stencil_from_forest(f)
+
+
+@pytest.mark.parametrize("dim,box,n_trees,periodic,shape,lmin,lmax", BRICKS,
+                         ids=["2x1", "periodic_xy", "3x2_periodic_x", "3d_2x1x1_periodic_y", "3d_1x2x2_periodic"])
+def test_brick_stencils_match_brute_force(dim, box, n_trees, periodic, shape, lmin, lmax):
+    """Bricks of trees, periodic or not: nodes (owner tree and coordinates,
+    in order), neighbour ids, face and transverse distances and wall bits
+    bit for bit against the brute-force brick restatement."""
+    f, tab = build_brick(dim, box, n_trees, periodic, shape, lmin, lmax)
+    assert tab.n_nodes < 6000
+    xyz, lvl, _ = f.leaves()
+    pos, leaf_tree, node_tree = f.trees()
+    nbr, d, dtr, wall, nodes = O.brute_stencil_brick(xyz, lvl, leaf_tree, pos, dim, box[0], box[1], n_trees,
+                                                      periodic)
+    ixyz, _ = f.nodes()
+    assert len(nodes) == tab.n_nodes
+    assert np.array_equal(node_tree, np.array([t for t, _ in nodes], dtype=np.int32))
+    assert np.array_equal(ixyz, np.array([c for _, c in nodes], dtype=np.int32))
+    assert np.array_equal(tab.nbr, nbr)
+    assert np.array_equal(tab.d, d)
+    assert np.array_equal(tab.d_tr, dtr)
+    assert np.array_equal(tab.wall, wall)
+    # periodic axes have no wall; every face distance is positive off the walls
+    for a in range(dim):
+        if periodic[a]:
+            assert not np.any(tab.wall & (3 << (2 * a)))
+    assert np.all(tab.d[:, tab.wall == 0] > 0)
+
+
+@pytest.mark.parametrize("dim,box,n_trees,periodic,shape,lmin,lmax", BRICKS[:1] + BRICKS[3:4], ids=["2d", "3d"])
+def test_brick_refinement_matches_refine_levelset_cf(dim, box, n_trees, periodic, shape, lmin, lmax):
+    f, _ = build_brick(dim, box, n_trees, periodic, shape, lmin, lmax)
+    xyz, lvl, _ = f.leaves()
+    pos, leaf_tree, _ = f.trees()
+    vmin, ext, _ = O.brick_geometry(dim, box[0], box[1], n_trees)
+    ML = O.MAXLEVEL[dim]
+    for q, lv, t in zip(xyz, lvl, leaf_tree):
+        tmin = [vmin[a][pos[t][a]] for a in range(dim)]
+        text = [ext[a][pos[t][a]] for a in range(dim)]
+        assert not O.refine_decision(q, int(lv), dim, lmin, lmax, 2.4, 0.0, shape, tree_min=tmin, tree_ext=text)
+        if lv > 0:
+            h = 1 << (ML - int(lv) + 1)
+            parent = [int(c) // h * h for c in q]
+            assert O.refine_decision(parent, int(lv) - 1, dim, lmin, lmax, 2.4, 0.0, shape, tree_min=tmin,
+                                     tree_ext=text)
+
+
+def test_brick_tree_order_and_leaf_order():
+    """Trees follow a Morton curve of their brick positions; leaves run tree
+    by tree, each tree's leaves in Morton order, and tile the domain."""
+    f, _ = build_brick(2, ([0.0, 0.0], [3.0, 2.0]), [3, 2], [0, 0], ls.Sphere((1.5, 1.0), 0.6), 1, 4)
+    pos, leaf_tree, _ = f.trees()
+    assert sorted(map(tuple, pos)) == [(i, j) for i in range(3) for j in range(2)]
+    for t in range(len(pos) - 1):
+        a, b = tuple(int(v) for v in pos[t]), tuple(int(v) for v in pos[t + 1])
+        assert O.quadrant_compare(a, b, 2) < 0
+    xyz, lvl, _ = f.leaves()
+    assert np.all(np.diff(leaf_tree) >= 0)
+    for i in range(len(xyz) - 1):
+        if leaf_tree[i] == leaf_tree[i + 1]:
+            assert O.quadrant_compare(tuple(int(v) for v in xyz[i]), tuple(int(v) for v in xyz[i + 1]), 2) < 0
+    vol = np.sum((1.0 / 2.0 ** lvl.astype(float)) ** 2)
+    assert abs(vol - 6.0) < 1e-12
+
+
+def test_brick_rejects_bad_arguments():
+    with pytest.raises(C.CaslError):
+        C.Forest(2, n_trees=[0, 1])
+    with pytest.raises(C.CaslError):
+        C.Forest(3, n_trees=[2000, 1, 1])
+    with pytest.raises(ValueError):
+        C.Forest(2, n_trees=[1, 1, 1])

@@ -12,7 +12,7 @@ forest (Forest.refine_levelset + build_nodes + construct_neighbors):
     one bit for bit;
   * the tiled encoding of the device-built stencil, made on the device from
     tables that never leave it, equals the host builder's encoding of the
-    host tables byte for byte (or neither exists: non-cubic cells);
+    host tables byte for byte (cubic and non-cubic cells);
   * the level sets refine the measured workloads (star2d, spheres3d at full
     size, gyroid3d at lmax 9) to the same grids.
 """
@@ -85,25 +85,37 @@ def test_device_grid_equals_host(cuda, case):
     assert ok, why
 
 
-@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5])], ids=["rect2d", "brick3d"])
+@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5]), (3, [1.5, 1.0, 0.75])],
+                         ids=["rect2d", "brick3d", "brick3d_nonpow2"])
 def test_device_tiling_noncubic_cells(cuda, dim, box_max):
-    """Cells that are not cubes: neither the host nor the device builds the
-    tiled encoding; the device-built stencil runs the two-pass path with the
-    host-built one's results."""
+    """Cells that are not cubes: on edges in power-of-two ratios the device
+    builds the same tiled encoding as the host (one scale per axis); on
+    others (1.5:1:0.75) neither builds one.  The device-built stencil gives
+    the host-built one's results on both paths."""
     import torch
-    phi = ls.Sphere([0.9] + [0.45] * (dim - 1), 0.3) if dim == 2 else ls.Sphere([0.9, 0.5, 0.25], 0.2)
+    phi = ls.Sphere([0.45 * e for e in box_max], 0.3 * min(box_max))
     f, tab = host_grid(dim, phi, 2, 7 if dim == 2 else 6, 2.4, 0.0, [0.0] * dim, box_max)
     g = C.DeviceGrid(dim, phi, 2, 7 if dim == 2 else 6, lip=2.4, box_min=[0.0] * dim, box_max=box_max)
     ds = g.device()
-    assert not ds.tile_info()["tiled"]
-    assert ds.path == "two_pass"
+    info = ds.tile_info()
+    dyadic = all(float(np.log2(e)).is_integer() for e in box_max)
+    assert info["tiled"] == dyadic, info
+    if dyadic:
+        assert info["n_irregular"] > 0, info
+    else:
+        assert "not cubes" in info["reason"], info
     ok, why = ds.tile_check(tab)
     assert ok, why
-    a = torch.from_numpy(f.sample(phi)).to(cuda)
-    b = a.clone()
-    C.DeviceStencil(tab).reinit(a, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0)
-    ds.reinit(b, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0)
-    assert torch.equal(a, b)
+    hs = C.DeviceStencil(tab)
+    for path in ("tiled", "two_pass"):
+        hs.set_path(path)
+        ds.set_path(path)
+        assert ds.path == (path if dyadic else "two_pass")
+        a = torch.from_numpy(f.sample(phi)).to(cuda)
+        b = a.clone()
+        hs.reinit(a, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0)
+        ds.reinit(b, 2, 2, 4, C.DBL_MAX, -C.DBL_MAX, 0)
+        assert torch.equal(a, b), path
 
 
 def test_device_grid_root_only_fails_like_host(cuda):

@@ -88,30 +88,120 @@ def test_reinit_fp64_matches_oracle(cuda, name, orders, path):
     assert np.array_equal(out, ref), rel_err(out, ref)
 
 
-@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5])], ids=["rect2d", "brick3d"])
+@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5]), (2, [3.0, 3.0]), (2, [3.0, 1.0]),
+                                          (3, [1.5, 1.0, 0.75])],
+                         ids=["rect2d", "brick3d", "square_3", "rect2d_3to1", "brick3d_nonpow2"])
 @pytest.mark.parametrize("path", ["tiled", "two_pass"])
 def test_noncubic_box_matches_oracle(cuda, dim, box_max, path):
-    """A single tree over a box whose edges differ (by powers of two): cells
-    are not cubes, so a hanging arm's transverse sums differ from its length.
-    The tile encoding assumes cubic cells and is not built for such a grid;
-    asking for the tiled path then runs the grid-wide tables, with the
-    oracle's results."""
+    """A single tree over a box whose edges differ: cells are not cubes, so a
+    hanging arm's transverse sums differ from its length.  Edges in power-of-
+    two ratios are tiled (one scale per axis); a cubic box of any edge is
+    tiled too; non-cubic cells on other edges (3:1, 1.5:1:0.75) keep the
+    grid-wide tables.  FP64 gives the oracle's bits on every path (on edges
+    that are not powers of two it runs the grid-wide kernels)."""
     import torch
     say(f"[gpu] non-cubic box {box_max} path={path}")
-    shape = ls.Sphere([0.9] + [0.45] * (dim - 1), 0.3) if dim == 2 else ls.Sphere([0.9, 0.5, 0.25], 0.2)
+    shape = ls.Sphere([0.45 * e for e in box_max], 0.3 * min(box_max))
     f = C.Forest(dim, [0.0] * dim, box_max).refine_levelset(shape, 2, 7 if dim == 2 else 6, lip=2.4).build_nodes()
     ngbd = C.NodeNeighbors(f).init_neighbors()
     phi0 = f.sample(shape)
     info = ngbd.device().tile_info()
-    assert not info["tiled"], info
+    dyadic = all(float(np.log2(e)).is_integer() for e in box_max)
+    assert info["tiled"] == (dyadic or len(set(box_max)) == 1), info
     ngbd.device().set_path(path)
-    assert ngbd.device().path == "two_pass"
+    assert ngbd.device().path == (path if info["tiled"] else "two_pass")
     it = 10
     ref = O.reinit(ngbd.tables, phi0, 2, 2, it)
     phi = torch.from_numpy(phi0).to(cuda)
     C.LevelSet(ngbd).reinitialize_within_range_of_phi_0(phi, 2, 2, C.DBL_MAX, -C.DBL_MAX, it)
     out = phi.cpu().numpy()
     say(f"    max |gpu - oracle| = {np.max(np.abs(out - ref)):.3e}")
+    assert np.array_equal(out, ref), rel_err(out, ref)
+    h = float(np.min(f.dxyz_min()))
+    mixed = torch.from_numpy(phi0).to(cuda)
+    C.LevelSet(ngbd, precision="mixed").reinitialize_within_range_of_phi_0(mixed, 2, 2, C.DBL_MAX, -C.DBL_MAX, it)
+    band = np.abs(out) < 3 * h
+    assert np.max(np.abs(mixed.cpu().numpy() - out)[band]) <= 1e-3 * h
+
+
+NONDYADIC = {
+    "2pi_square": (2, [0.0, 0.0], [2 * np.pi, 2 * np.pi], ls.Sphere((np.pi, 3.0), 2.0), 7),
+    "rect_0.3x0.1": (2, [-0.1, 0.0], [0.2, 0.1], ls.Sphere((0.05, 0.05), 0.04), 7),
+    "cube_0.3": (3, [0.0, 0.0, 0.0], [0.3, 0.3, 0.3], ls.Sphere((0.14, 0.15, 0.16), 0.08), 5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(NONDYADIC))
+def test_nondyadic_box_matches_oracle(cuda, name):
+    """Boxes whose cell sizes FP32 does not hold exactly ([0, 2 pi]^2 and
+    the like): the stencil keeps an FP64 copy of the distances and no tiled
+    encoding; second derivatives, reinitialization and advection in FP64
+    give the oracle's bits, the mixed path stays within 1e-3 h."""
+    import torch
+    dim, lo, hi, shape, lmax = NONDYADIC[name]
+    say(f"[gpu] non-dyadic box {name}")
+    f = C.Forest(dim, lo, hi).refine_levelset(shape, 2, lmax, lip=2.4).build_nodes()
+    ngbd = C.NodeNeighbors(f).init_neighbors()
+    tab = ngbd.tables
+    assert np.any(tab.d.astype(np.float32).astype(np.float64) != tab.d)
+    phi0 = f.sample(shape)
+    assert not ngbd.device().tile_info()["tiled"]
+    g = torch.from_numpy(phi0).to(cuda)
+    dd = ngbd.second_derivatives_central(g).cpu().numpy()
+    assert np.array_equal(dd, O.second_derivatives(tab, phi0)), rel_err(dd, O.second_derivatives(tab, phi0))
+    it = 10
+    ref = O.reinit(tab, phi0, 2, 2, it)
+    phi = torch.from_numpy(phi0).to(cuda)
+    C.LevelSet(ngbd).reinitialize_within_range_of_phi_0(phi, 2, 2, C.DBL_MAX, -C.DBL_MAX, it)
+    out = phi.cpu().numpy()
+    say(f"    {tab.n_nodes} nodes, max |gpu - oracle| = {np.max(np.abs(out - ref)):.3e}")
+    assert np.array_equal(out, ref), rel_err(out, ref)
+    h = float(np.min(f.dxyz_min()))
+    mixed = torch.from_numpy(phi0).to(cuda)
+    C.LevelSet(ngbd, precision="mixed").reinitialize_within_range_of_phi_0(mixed, 2, 2, C.DBL_MAX, -C.DBL_MAX, it)
+    band = np.abs(out) < 3 * h
+    assert np.max(np.abs(mixed.cpu().numpy() - out)[band]) <= 1e-3 * h
+    _, xyz = f.nodes()
+    vn = 0.5 + np.sin(4 * xyz[:, 0]) * np.cos(3 * xyz[:, 1])
+    p = torch.from_numpy(phi0).to(cuda)
+    dt = C.LevelSet(ngbd).advect_in_normal_direction(torch.from_numpy(vn).to(cuda), p)
+    assert dt == O.advect_cfl_dt(tab, vn)
+    assert np.array_equal(p.cpu().numpy(), O.advect(tab, phi0, vn, dt))
+
+
+BRICKS = {
+    "2x1_periodic_x": (2, ([0.0, 0.0], [2.0, 1.0]), [2, 1], [1, 0], ls.Sphere((0.1, 0.5), 0.3), 2, 7),
+    "periodic_xy": (2, ([0.0, 0.0], [1.0, 1.0]), [1, 1], [1, 1], ls.Star(center=(0.05, 0.95), r0=0.3), 2, 7),
+    "3x2_box": (2, ([-1.0, 0.0], [2.0, 2.0]), [3, 2], [0, 0], ls.Sphere((0.0, 1.0), 0.6), 2, 6),
+    "3d_2x1x1_periodic_y": (3, ([0.0, 0.0, 0.0], [2.0, 1.0, 1.0]), [2, 1, 1], [0, 1, 0],
+                            ls.Sphere((1.0, 0.1, 0.5), 0.3), 1, 5),
+    "3d_periodic_xyz": (3, ([0.0, 0.0, 0.0], [1.0, 1.0, 1.0]), [1, 2, 2], [1, 1, 1],
+                        ls.Sphere((0.05, 0.5, 0.95), 0.3), 1, 5),
+}
+
+
+@pytest.mark.parametrize("name", sorted(BRICKS))
+@pytest.mark.parametrize("path", ["tiled", "two_pass"])
+def test_brick_forest_matches_oracle(cuda, name, path):
+    """Tables of bricks of trees, periodic or not (arms that wrap around the
+    domain, neighbours across tree faces): FP64 on both paths gives the
+    oracle's bits, mixed stays within 1e-3 h of it."""
+    import torch
+    dim, box, n_trees, periodic, shape, lmin, lmax = BRICKS[name]
+    say(f"[gpu] brick {name} path={path}")
+    f = C.Forest(dim, box[0], box[1], n_trees=n_trees, periodic=periodic)
+    f.refine_levelset(shape, lmin, lmax, lip=2.4).build_nodes()
+    ngbd = C.NodeNeighbors(f).init_neighbors()
+    phi0 = f.sample(shape)
+    info = ngbd.device().tile_info()
+    assert info["tiled"], info
+    ngbd.device().set_path(path)
+    it = 10
+    ref = O.reinit(ngbd.tables, phi0, 2, 2, it)
+    phi = torch.from_numpy(phi0).to(cuda)
+    C.LevelSet(ngbd).reinitialize_within_range_of_phi_0(phi, 2, 2, C.DBL_MAX, -C.DBL_MAX, it)
+    out = phi.cpu().numpy()
+    say(f"    {ngbd.tables.n_nodes} nodes, max |gpu - oracle| = {np.max(np.abs(out - ref)):.3e}")
     assert np.array_equal(out, ref), rel_err(out, ref)
     h = float(np.min(f.dxyz_min()))
     mixed = torch.from_numpy(phi0).to(cuda)

@@ -48,9 +48,25 @@ def test_uniform_grid_has_no_hanging_faces():
     assert plan["n_irregular"] == 0 and plan["n_tiles"] >= 1
 
 
-@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5])])
-def test_noncubic_cells_are_not_tiled(dim, box_max):
-    shape = ls.Sphere([0.9] + [0.45] * (dim - 1), 0.3) if dim == 2 else ls.Sphere([0.9, 0.5, 0.25], 0.2)
+@pytest.mark.parametrize("dim,box_max", [(2, [2.0, 1.0]), (3, [2.0, 1.0, 0.5]), (2, [3.0, 3.0])],
+                         ids=["rect2d", "brick3d", "square_3"])
+def test_noncubic_or_nondyadic_cells_are_tiled(dim, box_max):
+    """Boxes whose edges differ by powers of two: cells are not cubes, so a
+    hanging arm's transverse extents differ from its length.  The encoding
+    keeps one scale per axis; the builder's record check (every decoded
+    distance, transverse offset and corner equal to the tables') passes,
+    hanging faces included.  A cubic box of any edge (3 x 3) is tiled too."""
+    shape = ls.Sphere([0.45 * e for e in box_max], 0.3 * min(box_max))
     f = C.Forest(dim, [0.0] * dim, box_max).refine_levelset(shape, 2, 6, lip=2.4).build_nodes()
-    with pytest.raises(C.CaslError):
+    plan = C.tile_plan(C.flat_stencil_from_forest(f))
+    assert plan["n_tiles"] >= 1 and plan["n_irregular"] > 0, plan
+
+
+@pytest.mark.parametrize("dim,box_max", [(2, [3.0, 1.0]), (3, [1.5, 1.0, 0.75])], ids=["rect_3to1", "brick_nonpow2"])
+def test_noncubic_nondyadic_cells_are_not_tiled(dim, box_max):
+    """Cells that are not cubes on edges that are not powers of two: no
+    tiled encoding (such grids run the grid-wide kernels)."""
+    shape = ls.Sphere([0.45 * e for e in box_max], 0.3 * min(box_max))
+    f = C.Forest(dim, [0.0] * dim, box_max).refine_levelset(shape, 2, 6, lip=2.4).build_nodes()
+    with pytest.raises(C.CaslError, match="not cubes"):
         C.tile_plan(C.flat_stencil_from_forest(f))

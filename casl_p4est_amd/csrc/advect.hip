@@ -61,7 +61,7 @@ __global__ void __launch_bounds__(TILE_BLOCK_F64) __attribute__((amdgpu_waves_pe
   TileCtx<D> c;
   c.load(A, tile);
   const int s = c.s, nT = c.nT, hb = c.hb, nH = c.nH, nH1 = c.nH1;
-  const bool ufast = A.geom.pow2 != 0;
+  const bool ufast = A.geom.pow2 != 0 && A.geom.cubic != 0;
   RecRegs trec[KT];
   int tslot[KT];
 #pragma unroll

@@ -180,16 +180,19 @@ dev::TileArgs<R> tile_args(const casl_stencil *st) {
   }
   ta.geom.cubic = 1;
   for (int a = 1; a < st->dim; ++a) ta.geom.cubic &= int(st->geom_base[a] == st->geom_base[0]);
-  ta.geom.exp_pack = 0;
-  ta.geom.lmax = 0;
-  if (ta.geom.pow2) {
+  ta.geom.d_pack = 0;
+  {
     int e = 0;
+    std::frexp(st->geom_base[0] / st->geom_unit[0], &e);
+    ta.geom.lmax = e - 1;  // base / unit = 2^lmax, exact for any base
+    int ex[3];
+    for (int a = 0; a < 3; ++a) std::frexp(st->geom_base[a], &ex[a]);
     for (int a = 0; a < 3; ++a) {
-      std::frexp(st->geom_base[a], &e);  // base = 0.5 * 2^e
-      ta.geom.exp_pack |= uint32_t((e - 1 + 128) & 255) << (8 * a);
+      const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
+      // cubic cells: zero; non-cubic ones come on a pow2 geometry only
+      const int d1 = ta.geom.cubic ? 0 : ex[t1] - ex[a], d2 = ta.geom.cubic ? 0 : ex[t2] - ex[a];
+      ta.geom.d_pack |= uint64_t(uint32_t((d1 + 128) & 255) | (uint32_t((d2 + 128) & 255) << 8)) << (16 * a);
     }
-    std::frexp(st->geom_base[0] / st->geom_unit[0], &e);  // base / unit = 2^lmax
-    ta.geom.lmax = e - 1;
   }
   ta.N = st->N;
   return ta;

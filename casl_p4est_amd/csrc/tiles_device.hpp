@@ -17,11 +17,13 @@ struct TileGeom {
   R inv_base[3];  // 1 / base, exact when pow2
   int pow2;       // every base[a] is a power of two: all arm lengths are, too
   int cubic;      // one scale on every axis (cubic cells)
-  // pow2 only: the exponents e_a of base[a] = 2^e_a, each + 128 in 8 bits
-  // (axis a at bits 8a), and the finest level (unit[a] = 2^(e_a - lmax));
-  // a run-time axis picks its exponent with a shift
-  uint32_t exp_pack;
-  int lmax;
+  int lmax;       // the finest level: unit[a] = base[a] * 2^-lmax
+  // Per face axis a (16 bits at 16a): e_t1 - e_a and e_t2 - e_a, each + 128
+  // in 8 bits, where base[t] = 2^e_t, for its transverse axes t1 (y for x,
+  // else x) and t2 (y for z, else z); zero differences when the cells are
+  // cubes (non-cubic cells come on a pow2 geometry only).  A run-time face
+  // axis picks them with one shift.
+  uint64_t d_pack;
   // pow2 only: the bit pattern (FP32) or high word (FP64) of base and 1/base,
   // whose exponent field a level nibble moves down or up
   uint32_t base_bits[3], inv_bits[3];
@@ -31,9 +33,6 @@ __device__ __forceinline__ double pow2neg(double, int l) { return __longlong_as_
 __device__ __forceinline__ float pow2neg(float, int l) { return __int_as_float((127 - l) << 23); }
 __device__ __forceinline__ double pow2pos(double, int l) { return __longlong_as_double((1023ll + l) << 52); }
 __device__ __forceinline__ float pow2pos(float, int l) { return __int_as_float((127 + l) << 23); }
-// 2^e for a normal-range exponent e
-__device__ __forceinline__ double pow2i(double, int e) { return __longlong_as_double((1023ll + e) << 52); }
-__device__ __forceinline__ float pow2i(float, int e) { return __int_as_float((127 + e) << 23); }
 
 // Face distances of a record: arm-cell sizes from the level nibbles, wall arms
 // -(d_opposite) - (second-degree cell size) (construct_neighbors :609-719).
@@ -113,11 +112,12 @@ struct RegStencil : DecodedDist<D, R> {
   }
 };
 
-// Stencil of a uniform node on a dyadic geometry (every arm cell a power of
-// two; the kernels take this accessor only when TileGeom::pow2): one corner
-// per face, the arms along axis a q_a = base_a * 2^-level long, no wall.  The per-face arrays
-// hold the same value so the shared stencil code reads them as it reads a
-// RegStencil's.
+// Stencil of a uniform node on a dyadic geometry of cubic cells (every arm
+// cell a power of two, one scale on every axis; the kernels take this
+// accessor only when TileGeom::pow2 and cubic, other geometries decode these
+// records as RegStencil): one corner per face, all arms q = base * 2^-level
+// long, no wall.  The per-face arrays hold the same value so the shared
+// stencil code reads them as it reads a RegStencil's.
 template <int D, class R>
 struct UniStencil {
   static constexpr bool REGULAR = true;
@@ -128,7 +128,7 @@ struct UniStencil {
   static constexpr int F = 2 * D;
   uint16_t id[F];
   R dist[F], q[F], iq[F];
-  R iq2[D];  // 1 / q_a^2 per axis
+  R iq2[D];  // 1 / q^2 (per axis for the shared second difference)
   __device__ __forceinline__ int32_t idx(int f, int, int64_t) const { return id[f]; }
   __device__ __forceinline__ R d(int f, int64_t) const { return dist[f]; }
   __device__ __forceinline__ R dt(int f, int k, int64_t) const { return (k & 1) ? q[f] : R(0); }
@@ -136,14 +136,16 @@ struct UniStencil {
   __device__ __forceinline__ int load(const uint32_t *w, const TileGeom<R> &g) {
     int slot, level;
     uni_unpack<D>(w, id, slot, level);
+    const R qq = g.base[0] * pow2neg(R(0), level), iqq = g.inv_base[0] * pow2pos(R(0), level);
+    const R iqq2 = iqq * iqq;
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-      const R qq = g.base[a] * pow2neg(R(0), level), iqq = g.inv_base[a] * pow2pos(R(0), level);
-      dist[2 * a] = dist[2 * a + 1] = qq;
-      q[2 * a] = q[2 * a + 1] = qq;
-      iq[2 * a] = iq[2 * a + 1] = iqq;
-      iq2[a] = iqq * iqq;
+    for (int f = 0; f < F; ++f) {
+      dist[f] = qq;
+      q[f] = qq;
+      iq[f] = iqq;
     }
+#pragma unroll
+    for (int a = 0; a < D; ++a) iq2[a] = iqq2;
     return slot;
   }
 };
@@ -174,11 +176,10 @@ struct HangStencil : DecodedDist<D, R> {
   static constexpr int NW = WORDS / 2;  // 32-bit words (the record is whole 8-byte words)
   uint16_t id[F];
   uint32_t e[NW];
-  bool pow2, cubic;
+  bool pow2;
   uint32_t geo;
   R base[D], inv_base[D];  // the geometry's (uniform values)
-  R unit0;                 // finest cell (of every axis when cubic)
-  uint32_t exp_pack;       // TileGeom::exp_pack and lmax (pow2)
+  uint64_t d_pack;         // TileGeom::d_pack and lmax
   int lmax;
   // The arm cell size, its inverse and the face distances of the axis of a
   // face f known only at run time, from the level nibbles and wall bits as
@@ -216,21 +217,17 @@ struct HangStencil : DecodedDist<D, R> {
     r.dp = dO;
     r.dm = de;
     r.iqp = ib * pow2pos(R(0), lo);
-    if (!cubic) {
-      // first transverse axis: y for x, else x; second: y for z, else z
-      const int t1 = a == 0 ? 1 : 0, t2 = a == 2 ? 1 : 2;
-      const int e1 = int((exp_pack >> (8 * t1)) & 255u) - 128, e2 = int((exp_pack >> (8 * t2)) & 255u) - 128;
-      r.qt1 = pow2i(R(0), e1 - l);
-      r.iqt1 = pow2i(R(0), l - e1);
-      r.ut1 = pow2i(R(0), e1 - lmax);
-      r.qt2 = pow2i(R(0), e2 - l);
-      r.iqt2 = pow2i(R(0), l - e2);
-      r.ut2 = pow2i(R(0), e2 - lmax);
-    } else {
-      r.qt1 = r.qt2 = r.qf;
-      r.iqt1 = r.iqt2 = r.iqf;
-      r.ut1 = r.ut2 = unit0;
-    }
+    // the transverse axes' scales are base * 2^d, d = e_t - e_a (0 for
+    // cubic cells, whatever the base): exact power-of-two scalings
+    const uint32_t dp = uint32_t(d_pack >> (16 * a));
+    const int d1 = int(dp & 255u) - 128, d2 = int((dp >> 8) & 255u) - 128;
+    const R ua = ldexp(b, -lmax);
+    r.qt1 = ldexp(r.qf, d1);
+    r.iqt1 = ldexp(r.iqf, -d1);
+    r.ut1 = ldexp(ua, d1);
+    r.qt2 = ldexp(r.qf, d2);
+    r.iqt2 = ldexp(r.iqf, -d2);
+    r.ut2 = ldexp(ua, d2);
     return r;
   }
   __device__ __forceinline__ uint32_t word(int i) const { return (e[i >> 1] >> (16 * (i & 1))) & 0xffffu; }
@@ -281,15 +278,13 @@ struct HangStencil : DecodedDist<D, R> {
       e[2 * i + 1] = v.y;
     }
     pow2 = g.pow2 != 0;
-    cubic = g.cubic != 0;
     geo = rec.geo;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
       base[k] = g.base[k];
       inv_base[k] = g.inv_base[k];
     }
-    unit0 = g.unit[0];
-    exp_pack = g.exp_pack;
+    d_pack = g.d_pack;
     lmax = g.lmax;
     this->decode(rec.geo, g);
   }
@@ -499,7 +494,7 @@ __device__ __forceinline__ void tile_stage(const TileArgs<R> &A, int tile, R *__
   TileCtx<D> c;
   c.load(A, tile);
   const int s = c.s, nT = c.nT, hb = c.hb, nH = c.nH, nH1 = c.nH1;
-  const bool ufast = A.geom.pow2 != 0;
+  const bool ufast = A.geom.pow2 != 0 && A.geom.cubic != 0;
   RecRegs trec[KT];
   uint32_t tsf[KT];  // LDS slot of a non-uniform record (low 16 bits) and flags (high 16 bits)
   // the phi_n averaged with in the second RK2 stage: loaded before the first
@@ -656,7 +651,7 @@ __device__ __forceinline__ void tile_prep(const TileArgs<double> &A, int tile, d
   TileCtx<D> c;
   c.load(A, tile);
   const int s = c.s, nT = c.nT, hb = c.hb, nH = c.nH, nH1 = c.nH1;
-  const bool ufast = A.geom.pow2 != 0;
+  const bool ufast = A.geom.pow2 != 0 && A.geom.cubic != 0;
   for (int j = threadIdx.x; j < nT; j += BLOCK) phis[j] = A.phin[s + j];
   for (int j = threadIdx.x; j < nH; j += BLOCK) phis[nT + j] = A.phin[A.halo[hb + j]];
   __syncthreads();

@@ -51,6 +51,27 @@ __host__ __device__ inline float bf2f(uint16_t h) {
   return f;
 }
 
+// The GEMM epilogue of one 32x32 wave tile at (tm, tn): bias, ReLU or the
+// ReLU'-mask, FP32 and/or BF16 stores.
+__device__ __forceinline__ void gemm_store(const f32x16 &acc, int M, int N, int tm, int tn, int lane,
+                                           const float *__restrict__ bias, const uint16_t *__restrict__ mask, int ldm,
+                                           int relu, float *__restrict__ C, uint16_t *__restrict__ C16, int ldc) {
+  const int li = lane & 31;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = tm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    const int col = tn + li;
+    if (row < M && col < N) {
+      float v = acc[r];
+      if (bias) v += bias[col];
+      if (relu) v = v > 0.f ? v : 0.f;
+      if (mask && !(bf2f(mask[size_t(row) * ldm + col]) > 0.f)) v = 0.f;
+      if (C) C[size_t(row) * ldc + col] = v;
+      if (C16) C16[size_t(row) * ldc + col] = f2bf(v);
+    }
+  }
+}
+
 // C[M x N] = op(A)[M x K] op(B)[K x N] (+ bias[N]) then ReLU or ReLU'-mask.
 //   op(A)(i, k) = TA ? A[k*lda + i] : A[i*lda + k]
 //   op(B)(k, j) = TB ? B[j*ldb + k] : B[k*ldb + j]
@@ -107,19 +128,87 @@ __global__ void __launch_bounds__(256)
     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
   }
   float *C = C32 ? C32 + size_t(blockIdx.z) * M * ldc : nullptr;
+  gemm_store(acc, M, N, tm, tn, lane, bias, mask, ldm, relu, C, C16, ldc);
+}
+
+// k_gemm with LDS-staged panels, for M and N multiples of 64 and K ranges
+// (K and kc) multiples of 64 with 16-byte aligned operands: per K chunk of
+// 64 the block's op(A) panel [64 x 64] and op(B) panel [64 x 64] are loaded
+// with 16-byte vectors, all of a thread's loads in flight together, into
+// LDS (k contiguous per row of op(A) and per column of op(B); transposed
+// layouts are transposed on the way in), double-buffered so the next
+// chunk's loads overlap this chunk's MFMAs.  Each wave then reads its
+// operands from LDS.  The MFMA sequence (k ascending by 16) and the operand
+// layout are k_gemm's, so the results are the same bits.
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256)
+    k_gemm_lds(int M, int N, int K, int kc, const uint16_t *__restrict__ A, int lda, const uint16_t *__restrict__ B,
+               int ldb, const float *__restrict__ bias, const uint16_t *__restrict__ mask, int ldm, int relu,
+               float *__restrict__ C32, uint16_t *__restrict__ C16, int ldc) {
+  constexpr int KC = 64, LS = KC + 8;  // chunk, LDS row stride in bf16 (144 bytes: 16-byte aligned rows)
+  __shared__ __align__(16) uint16_t sa[2][64 * LS], sb[2][64 * LS];
+  const int tid = int(threadIdx.x), wave = tid >> 6, lane = tid & 63;
+  const int bm = int(blockIdx.y) * 64, bn = int(blockIdx.x) * 64;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int li = lane & 31, lk = (lane >> 5) * 8;
+  const int kb = int(blockIdx.z) * kc, ke = min(K, kb + kc);
+  const int nch = (ke - kb) / KC;
+  // vector v of a panel (2 per thread): row v / 8 (of op(A), of op(B)^T, or
+  // of the stored k-major matrix when transposed), 8 elements at (v % 8) * 8
+  uint4 ra[2], rb[2];
+  auto gload = [&](int k0) {
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = tm + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    const int col = tn + li;
-    if (row < M && col < N) {
-      float v = acc[r];
-      if (bias) v += bias[col];
-      if (relu) v = v > 0.f ? v : 0.f;
-      if (mask && !(bf2f(mask[size_t(row) * ldm + col]) > 0.f)) v = 0.f;
-      if (C) C[size_t(row) * ldc + col] = v;
-      if (C16) C16[size_t(row) * ldc + col] = f2bf(v);
+    for (int h = 0; h < 2; ++h) {
+      const int v = tid + 256 * h, r = v >> 3, c = (v & 7) * 8;
+      ra[h] = TA ? *reinterpret_cast<const uint4 *>(A + size_t(k0 + r) * lda + bm + c)
+                 : *reinterpret_cast<const uint4 *>(A + size_t(bm + r) * lda + k0 + c);
+      rb[h] = TB ? *reinterpret_cast<const uint4 *>(B + size_t(bn + r) * ldb + k0 + c)
+                 : *reinterpret_cast<const uint4 *>(B + size_t(k0 + r) * ldb + bn + c);
     }
+  };
+  auto lstore = [&](int buf) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int v = tid + 256 * h, r = v >> 3, c = (v & 7) * 8;
+      if (TA) {
+        const uint16_t *e = reinterpret_cast<const uint16_t *>(&ra[h]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sa[buf][(c + t) * LS + r] = e[t];
+      } else {
+        *reinterpret_cast<uint4 *>(&sa[buf][r * LS + c]) = ra[h];
+      }
+      if (TB) {
+        *reinterpret_cast<uint4 *>(&sb[buf][r * LS + c]) = rb[h];
+      } else {
+        const uint16_t *e = reinterpret_cast<const uint16_t *>(&rb[h]);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sb[buf][(c + t) * LS + r] = e[t];
+      }
+    }
+  };
+  f32x16 acc;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  if (nch > 0) {
+    gload(kb);
+    lstore(0);
   }
+  __syncthreads();
+  for (int ch = 0; ch < nch; ++ch) {
+    const int buf = ch & 1;
+    if (ch + 1 < nch) gload(kb + (ch + 1) * KC);
+#pragma unroll
+    for (int ks = 0; ks < KC / 16; ++ks) {
+      const uint4 ua = *reinterpret_cast<const uint4 *>(&sa[buf][(wm + li) * LS + ks * 16 + lk]);
+      const uint4 ub = *reinterpret_cast<const uint4 *>(&sb[buf][(wn + li) * LS + ks * 16 + lk]);
+      const bf16x8 a = __builtin_bit_cast(bf16x8, ua), b = __builtin_bit_cast(bf16x8, ub);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (ch + 1 < nch) lstore(buf ^ 1);
+    __syncthreads();
+  }
+  float *C = C32 ? C32 + size_t(blockIdx.z) * M * ldc : nullptr;
+  gemm_store(acc, M, N, bm + wm, bn + wn, lane, bias, mask, ldm, relu, C, C16, ldc);
 }
 
 // out[e] = sum_z part[z*n + e], z = 0 .. splits-1 in order
@@ -204,9 +293,15 @@ void gemm(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, in
           int splits = 1) {
   const int kc = splits > 1 ? ((K + splits - 1) / splits + 15) / 16 * 16 : K;
   dim3 grid(unsigned((N + 63) / 64), unsigned((M + 63) / 64), unsigned(splits));
+  const bool staged = M % 64 == 0 && N % 64 == 0 && K % 64 == 0 && kc % 64 == 0 && lda % 8 == 0 && ldb % 8 == 0 &&
+                      (reinterpret_cast<uintptr_t>(A) & 15) == 0 && (reinterpret_cast<uintptr_t>(B) & 15) == 0;
   prof_begin("k_gemm", s);
-  hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, s, M, N, K, kc, A, lda, B, ldb, bias, mask, ldm, relu,
-                     C32, C16, ldc);
+  if (staged)
+    hipLaunchKernelGGL((k_gemm_lds<TA, TB>), grid, dim3(256), 0, s, M, N, K, kc, A, lda, B, ldb, bias, mask, ldm,
+                       relu, C32, C16, ldc);
+  else
+    hipLaunchKernelGGL((k_gemm<TA, TB>), grid, dim3(256), 0, s, M, N, K, kc, A, lda, B, ldb, bias, mask, ldm, relu,
+                       C32, C16, ldc);
   prof_end("k_gemm", s);
   HIP_CHECK(hipGetLastError());
 }

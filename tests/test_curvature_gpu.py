@@ -61,6 +61,45 @@ def test_bf16_gemm_layout(cuda, M, N, K, ta, tb):
     assert np.array_equal(C16.cpu().numpy().view(np.uint16), O.bf16_bits(got))
 
 
+@pytest.mark.parametrize("M,N,K", [(128, 128, 128), (256, 64, 192), (64, 128, 1024)])
+@pytest.mark.parametrize("ta,tb", [(0, 0), (1, 0), (0, 1)])
+def test_bf16_gemm_lds_path_same_bits(cuda, M, N, K, ta, tb):
+    """Shapes in multiples of 64 take the LDS-staged kernel; the same
+    matrices stored with a padded leading dimension (not a multiple of 8)
+    take the direct one: the same bits, with bias and ReLU too."""
+    import torch
+    from casl_p4est_amd import _lib as L
+    from casl_p4est_amd.level_set import _stream_ptr
+    import ctypes
+    say(f"[gemm lds] M={M} N={N} K={K} ta={ta} tb={tb}")
+    rng = np.random.default_rng(M + 3 * N + 5 * K + 7 * ta + 11 * tb)
+    A = O.bf16(rng.standard_normal((M, K)).astype(np.float32))
+    B = O.bf16(rng.standard_normal((K, N)).astype(np.float32))
+    a_store = A.T.copy() if ta else A
+    b_store = B.T.copy() if tb else B
+    bias = torch.from_numpy(rng.standard_normal(N).astype(np.float32)).cuda()
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    outs = []
+    for pad in (0, 1):
+        sa = np.pad(a_store, ((0, 0), (0, pad)))
+        sb = np.pad(b_store, ((0, 0), (0, pad)))
+        dA, dB = dev16(torch, sa), dev16(torch, sb)
+        C32 = torch.full((M, N), np.nan, dtype=torch.float32, device=cuda)
+        C16 = torch.zeros((M, N), dtype=torch.int16, device=cuda)
+        L.call("casl_bf16_gemm", M, N, K, p(dA), sa.shape[1], ta, p(dB), sb.shape[1], tb, None, 0, p(C32), None, N,
+               _stream_ptr(torch))
+        plain = C32.cpu().numpy()
+        L.call("casl_bf16_gemm", M, N, K, p(dA), sa.shape[1], ta, p(dB), sb.shape[1], tb, p(bias), 1, p(C32),
+               p(C16), N, _stream_ptr(torch))
+        outs.append((plain, C32.cpu().numpy(), C16.cpu().numpy()))
+    for x, y in zip(outs[0], outs[1]):
+        assert np.array_equal(x.view(np.uint32) if x.dtype == np.float32 else x,
+                              y.view(np.uint32) if y.dtype == np.float32 else y)
+    exact = A.astype(np.float64) @ B.astype(np.float64)
+    tol = 4 * K * 2.0 ** -24 * (np.abs(A).astype(np.float64) @ np.abs(B).astype(np.float64)) + 1e-30
+    assert np.all(np.abs(outs[0][0] - exact) <= tol)
+
+
 def _net_and_data(seed, n, dims=(10, 128, 128, 128, 128, 1)):
     from casl_p4est_amd.curvature import CurvatureNet
     rng = np.random.default_rng(seed)

@@ -139,10 +139,15 @@ def run_batch(args, torch):
     ms_b, cnt = kp["k_batch"]
     alg = b_alg(2, 4) * N * iters
     achieved = alg / (ms_b / cnt * 1e-3)
+    tr = pmc_traffic("k_batch", "batch2d", N)
     roofline = dict(bound="hbm", kernel="k_batch", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s",
-                    frac=achieved / HBM_PEAK, traffic=None, alg_bytes_per_launch=alg, avg_launch_ms=ms_b / cnt,
+                    frac=achieved / HBM_PEAK, traffic=tr["bytes_per_launch"] if tr else None,
+                    alg_bytes_per_launch=alg, avg_launch_ms=ms_b / cnt,
                     note="one launch = all 20 iterations of every tree; working set L2/MALL resident, so this "
                          "is effective bandwidth")
+    if tr:
+        roofline["traffic_source"] = tr["source"]
+        roofline["traffic_over_alg"] = tr["bytes_per_launch"] / alg
     cpu = None
     if not args.no_cpu:
         from oracle import pyoracle as O
@@ -544,7 +549,12 @@ def extras(args, W, lsr, phi0, out_mixed, dev, torch, C):
     log(f"[bench] advect_in_normal_direction: {advect['value']:.3e} node-steps/s, "
         f"{advect['ms_per_step']:.2f} ms per step ({other}: {1e3 * t_other / args.steps:.2f} ms)")
 
-    # the same grid built on the device (refinement, node numbering, construct_neighbors)
+    # the same grid built on the device (refinement, node numbering, construct_neighbors),
+    # next to the resident workload: release what the advection run held first
+    del vn, adv
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
     def build_on_device():
         g = C.DeviceGrid(dim, W["shape"], W["lmin"], W["lmax"], lip=2.4)
         torch.cuda.synchronize()
@@ -554,17 +564,30 @@ def extras(args, W, lsr, phi0, out_mixed, dev, torch, C):
         if g.sizes() != W["forest"].sizes():
             raise RuntimeError("device-built grid differs from the host-built one")
         del g
-        t_grid = []
+        t_grid, t_tile = [], []
         for _ in range(max(1, args.steps)):
             t0 = time.perf_counter()
             g = build_on_device()
             t_grid.append(time.perf_counter() - t0)
+            # the device stencil of those tables, its tiled encoding built on
+            # the device too (no table leaves HBM)
+            t0 = time.perf_counter()
+            info = g.device().tile_info()
+            torch.cuda.synchronize()
+            t_tile.append(time.perf_counter() - t0)
             del g
+        ref_info = W["ngbd"].device().tile_info()
+        if {k: info[k] for k in ("tiled", "n_tiles", "n_halo", "n_records", "n_irregular")} != \
+                {k: ref_info[k] for k in ("tiled", "n_tiles", "n_halo", "n_records", "n_irregular")}:
+            raise RuntimeError("device-built tiled encoding differs in its counts from the host-built one")
         host_s = sum(W["build_s"].values())
-        grid = {"device_s": min(t_grid), "host_s": host_s, "host_phases_s": W["build_s"],
-                "host_threads": workload_threads(), "speedup": host_s / min(t_grid),
-                "note": "refine_levelset_cf + node numbering + construct_neighbors, tables left in HBM"}
-        log(f"[bench] grid construction: device {grid['device_s']:.3f} s, CPU {host_s:.2f} s "
+        grid = {"device_s": min(t_grid), "device_stencil_and_tiling_s": min(t_tile), "host_s": host_s,
+                "host_phases_s": W["build_s"], "host_threads": workload_threads(),
+                "speedup": host_s / min(t_grid),
+                "note": "refine_levelset_cf + node numbering + construct_neighbors, tables left in HBM; then the "
+                        "device stencil with its tiled encoding built on the device"}
+        log(f"[bench] grid construction: device {grid['device_s']:.3f} s + stencil and tiling "
+            f"{grid['device_stencil_and_tiling_s']:.3f} s, CPU {host_s:.2f} s "
             f"({grid['host_threads']} threads), x{grid['speedup']:.1f}")
     except C.CaslError as e:  # e.g. out of device memory next to the resident workload
         grid = {"error": str(e)}

@@ -245,19 +245,38 @@ __global__ void k_mse_seed(const float *__restrict__ y, const float *__restrict_
   if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
 }
 
-// column sums of an [n x m] FP32 matrix (bias gradients)
-__global__ void k_colsum(const float *__restrict__ x, int n, int m, float *__restrict__ out) {
+// Column sums of an [n x m] FP32 matrix (bias gradients) in two passes with
+// a fixed order.  k_colsum_part: block b sums rows [b * COLSUM_ROWS, ...)
+// per column with coalesced row reads (consecutive threads take consecutive
+// columns; 256 / m row lanes when m < 256, added in lane order) into
+// part[b * m + c]; k_colsum_fin adds the blocks' partials in block order.
+constexpr int COLSUM_ROWS = 256;
+__global__ void __launch_bounds__(256)
+    k_colsum_part(const float *__restrict__ x, int n, int m, float *__restrict__ part) {
   __shared__ float red[256];
-  const int c = blockIdx.x;
-  float s = 0.f;
-  for (int r = threadIdx.x; r < n; r += blockDim.x) s += x[size_t(r) * m + c];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
+  const int t = int(threadIdx.x), b = int(blockIdx.x);
+  const int r0 = b * COLSUM_ROWS, r1 = min(n, r0 + COLSUM_ROWS);
+  for (int c0 = 0; c0 < m; c0 += 256) {
+    const int cw = min(m - c0, 256), lanes = 256 / cw, lane = t / cw, c = c0 + t % cw;
+    float s = 0.f;
+    if (lane < lanes)
+      for (int r = r0 + lane; r < r1; r += lanes) s += x[size_t(r) * m + c];
+    red[t] = s;
+    __syncthreads();
+    if (t < cw) {
+      float tot = red[t];
+      for (int l = 1; l < lanes; ++l) tot += red[l * cw + t];
+      part[size_t(b) * m + c0 + t] = tot;
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[c] = red[0];
+}
+__global__ void k_colsum_fin(const float *__restrict__ part, int nb, int m, float *__restrict__ out) {
+  const int c = int(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (c >= m) return;
+  float s = part[c];
+  for (int b = 1; b < nb; ++b) s += part[size_t(b) * m + c];
+  out[c] = s;
 }
 
 // Adam on FP32 masters, refreshes the BF16 copy (may be null for biases)
@@ -329,6 +348,7 @@ struct casl_mlp {
   DevBuf<uint16_t> d16a, d16b;      // BF16 back-propagated gradients
   DevBuf<float> y32, loss;
   DevBuf<float> part;               // split-K partial sums of weight gradients
+  DevBuf<float> colpart;            // per-block column sums of bias gradients
   std::vector<size_t> offA;
   int maxw = 0;
   int64_t step = 0;
@@ -690,7 +710,12 @@ int casl_mlp_backward(casl_mlp_t *m, const float *x, const float *t, int n, floa
     for (int l = m->L - 1; l >= 0; --l) {
       const int in = m->dim[l], o = m->dim[l + 1];
       // bias gradient from the FP32 dZ_l in z32
-      hipLaunchKernelGGL(k_colsum, dim3(o), dim3(256), 0, s, m->z32.p, n, o, m->gb.p + m->offb[l]);
+      const int nb = (n + COLSUM_ROWS - 1) / COLSUM_ROWS;
+      m->colpart.ensure(size_t(std::max(nb, 1)) * o);
+      hipLaunchKernelGGL(k_colsum_part, dim3(unsigned(std::max(nb, 1))), dim3(256), 0, s, m->z32.p, n, o,
+                         m->colpart.p);
+      hipLaunchKernelGGL(k_colsum_fin, dim3(blocks(o)), dim3(256), 0, s, m->colpart.p, std::max(nb, 1), o,
+                         m->gb.p + m->offb[l]);
       HIP_CHECK(hipGetLastError());
       const uint16_t *Aprev = m->act16.p + m->offA[l];  // [n x in]
       // dW_l = A_{l-1}^T dZ_l : (in x n)(n x o)

@@ -10,7 +10,8 @@ Level-set functions for refinement and sampling live in .levelset.
 from . import levelset
 from ._lib import CaslError
 from .batch import TreeBatch
-from .forest import DeviceGrid, FlatStencil, Forest, NodeNeighbors, flat_stencil_from_forest
+from .forest import (DeviceGrid, FlatStencil, Forest, NodeNeighbors, device_cache_held, device_cache_set_limit,
+                     device_cache_trim, flat_stencil_from_forest)
 from .level_set import (DBL_MAX, DeviceStencil, LevelSet, compute_mean_curvature, compute_normals,
                         my_p4est_level_set_t, tile_plan)
 
@@ -18,6 +19,6 @@ my_p4est_node_neighbors_t = NodeNeighbors
 
 __all__ = [
     "CaslError", "DBL_MAX", "DeviceGrid", "DeviceStencil", "FlatStencil", "Forest", "LevelSet", "NodeNeighbors", "TreeBatch",
-    "compute_mean_curvature", "compute_normals",
+    "compute_mean_curvature", "compute_normals", "device_cache_held", "device_cache_set_limit", "device_cache_trim",
     "flat_stencil_from_forest", "levelset", "my_p4est_level_set_t", "my_p4est_node_neighbors_t", "tile_plan",
 ]

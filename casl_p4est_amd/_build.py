@@ -36,7 +36,7 @@ HIP_FLAGS = [
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", INCLUDE, "-I", CSRC]
 
 HIP_SOURCES = ["reinit.hip", "batch.hip", "mlp.hip", "curv.hip", "derivatives.hip", "advect.hip", "gpu_forest.hip",
-               "curv_grid.hip", "noalloc.hip", "tiles_gpu.hip"]
+               "curv_grid.hip", "noalloc.hip", "tiles_gpu.hip", "devmem.hip"]
 HOST_SOURCES = ["forest.cpp", "tiles.cpp"]
 
 

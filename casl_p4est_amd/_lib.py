@@ -103,6 +103,9 @@ SIGNATURES = {
     "casl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "casl_profile_reset": (ctypes.c_int, []),
     "casl_profile_query": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double), c_int64_p]),
+    "casl_device_cache_set_limit": (ctypes.c_int, [ctypes.c_size_t]),
+    "casl_device_cache_trim": (ctypes.c_int, []),
+    "casl_device_cache_held": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)]),
     "casl_batch_create": (ctypes.c_int, [ctypes.POINTER(FlatStencil), ctypes.c_int32,
                                          ctypes.POINTER(ctypes.c_void_p)]),
     "casl_batch_destroy": (ctypes.c_int, [ctypes.c_void_p]),

@@ -493,7 +493,7 @@ struct casl_device_grid {
   double bmin[3] = {0, 0, 0}, bmax[3] = {1, 1, 1};
   double zdt = 0;
   casl::DevBuf<DQuad> leaves;
-  casl::DevBuf<int32_t> ixyz, local, nbr;
+  casl::DevBuf<int32_t> ixyz, nbr;
   casl::DevBuf<double> dface, dtr;
   casl::DevBuf<uint8_t> wall;
 };
@@ -550,36 +550,44 @@ void build_grid(casl_device_grid *g, int min_lvl, int max_lvl, double lip, doubl
     q2.ensure(size_t(L2));
     hipLaunchKernelGGL(k_emit<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, q.p, L, cnt.p, off.p, q2.p);
     HIP_CHECK(hipGetLastError());
-    std::swap(q.p, q2.p);
-    std::swap(q.n, q2.n);
+    q.swap(q2);
     L = L2;
     finest = pass + 1;
   }
-  // node numbering
+  HIP_CHECK(hipStreamSynchronize(s));  // the last emit has finished with q2, cnt and off
+  q2.release();
+  cnt.release();
+  off.release();
+  // node numbering: sort the leaves' corner keys (in place, between two
+  // buffers) and keep the distinct ones
   const int64_t K = L * CH;  // may exceed 2^31: the sort and unique take 64-bit counts
-  casl::DevBuf<uint64_t> keys, sorted, nodes;
-  casl::DevBuf<int64_t> nsel;
-  keys.ensure(size_t(K));
-  sorted.ensure(size_t(K));
-  nodes.ensure(size_t(K));
-  nsel.ensure(1);
-  hipLaunchKernelGGL(k_corner_keys<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, q.p, L, keys.p);
-  HIP_CHECK(hipGetLastError());
-  const int end_bit = D * GG::MAXLEVEL;
-  cub_call(s, [&](void *t, size_t &b) {
-    return hipcub::DeviceRadixSort::SortKeys(t, b, keys.p, sorted.p, K, 0, end_bit, s);
-  });
-  cub_call(s, [&](void *t, size_t &b) {
-    return hipcub::DeviceSelect::Unique(t, b, sorted.p, nodes.p, nsel.p, K, s);
-  });
   int64_t N = 0;
-  HIP_CHECK(hipMemcpyAsync(&N, nsel.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
-  HIP_CHECK(hipStreamSynchronize(s));
-  g->local.ensure(size_t(K));
-  hipLaunchKernelGGL(k_local_nodes, dim3(grid_for(K)), dim3(BLOCK), 0, s, keys.p, K, nodes.p, N, g->local.p);
-  g->ixyz.ensure(size_t(N) * D);
-  hipLaunchKernelGGL(k_node_ixyz<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, nodes.p, N, g->ixyz.p);
-  HIP_CHECK(hipGetLastError());
+  casl::DevBuf<int32_t> local;
+  {
+    casl::DevBuf<uint64_t> ka, kb;
+    casl::DevBuf<int64_t> nsel;
+    ka.ensure(size_t(K));
+    kb.ensure(size_t(K));
+    nsel.ensure(1);
+    hipLaunchKernelGGL(k_corner_keys<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, q.p, L, ka.p);
+    HIP_CHECK(hipGetLastError());
+    const int end_bit = D * GG::MAXLEVEL;
+    hipcub::DoubleBuffer<uint64_t> db(ka.p, kb.p);
+    cub_call(s, [&](void *t, size_t &b) { return hipcub::DeviceRadixSort::SortKeys(t, b, db, K, 0, end_bit, s); });
+    uint64_t *sorted = db.Current(), *nodes = db.Alternate();
+    cub_call(s, [&](void *t, size_t &b) { return hipcub::DeviceSelect::Unique(t, b, sorted, nodes, nsel.p, K, s); });
+    HIP_CHECK(hipMemcpyAsync(&N, nsel.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    HIP_CHECK(hipStreamSynchronize(s));
+    // each leaf corner's node: the corner keys again, in the sorted copy's
+    // place, looked up among the distinct ones
+    hipLaunchKernelGGL(k_corner_keys<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, q.p, L, sorted);
+    local.ensure(size_t(K));
+    hipLaunchKernelGGL(k_local_nodes, dim3(grid_for(K)), dim3(BLOCK), 0, s, sorted, K, nodes, N, local.p);
+    g->ixyz.ensure(size_t(N) * D);
+    hipLaunchKernelGGL(k_node_ixyz<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, nodes, N, g->ixyz.p);
+    HIP_CHECK(hipGetLastError());
+    HIP_CHECK(hipStreamSynchronize(s));  // the key buffers are freed here
+  }
   // neighbours
   casl::DevBuf<int32_t> cell;
   casl::DevBuf<uint64_t> leaf_keys;
@@ -589,7 +597,7 @@ void build_grid(casl_device_grid *g, int min_lvl, int max_lvl, double lip, doubl
   bad.ensure(1);
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(N * CH)), dim3(BLOCK), 0, s, cell.p, N * CH, -2);
   hipLaunchKernelGGL(k_fill_i32, dim3(1), dim3(1), 0, s, reinterpret_cast<int32_t *>(bad.p), int64_t(1), 0);
-  hipLaunchKernelGGL(k_scatter_cells<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, g->local.p, L, cell.p);
+  hipLaunchKernelGGL(k_scatter_cells<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, local.p, L, cell.p);
   hipLaunchKernelGGL(k_leaf_keys<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, q.p, L, leaf_keys.p);
   hipLaunchKernelGGL(k_locate<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, g->ixyz.p, N, leaf_keys.p, L, cell.p);
   HIP_CHECK(hipGetLastError());
@@ -598,17 +606,16 @@ void build_grid(casl_device_grid *g, int min_lvl, int max_lvl, double lip, doubl
   g->dtr.ensure(size_t(GG::F) * GG::T * N);
   g->wall.ensure(size_t(N));
   NbrOut<D> o{g->nbr.p, g->dface.p, g->dtr.p, g->wall.p, bad.p};
-  hipLaunchKernelGGL(k_construct<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, G, q.p, g->local.p, g->ixyz.p, cell.p, N,
+  hipLaunchKernelGGL(k_construct<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, G, q.p, local.p, g->ixyz.p, cell.p, N,
                      o);
   HIP_CHECK(hipGetLastError());
   int hbad = 0;
   HIP_CHECK(hipMemcpyAsync(&hbad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_CHECK(hipStreamSynchronize(s));
   if (hbad) throw std::runtime_error("construct_neighbors: a wall arm has no second-degree cell");
-  // keep the leaves; the finest level present
-  g->leaves.ensure(size_t(L));
-  HIP_CHECK(hipMemcpyAsync(g->leaves.p, q.p, size_t(L) * sizeof(DQuad), hipMemcpyDeviceToDevice, s));
-  HIP_CHECK(hipStreamSynchronize(s));
+  // keep the leaves (the refinement's buffer, handed over); the finest level present
+  g->leaves.release();
+  g->leaves.swap(q);
   g->L = L;
   g->N = N;
   g->max_level = finest;

@@ -11,6 +11,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/casl_reinit.h"
@@ -31,30 +32,40 @@ struct hip_error : std::runtime_error {
       throw ::casl::hip_error(std::string(#expr) + " failed: " + hipGetErrorString(_e));        \
   } while (0)
 
+// The library's device allocations (devmem.hip): hipMalloc, or a block of
+// the opt-in cache of released blocks.  device_alloc returns the block's
+// size (at least bytes) and device; device_free takes both back.
+void *device_alloc(size_t bytes, size_t *granted, int *device);
+void device_free(void *p, size_t bytes, int device);
+
 template <class T>
 struct DevBuf {
   T *p = nullptr;
   size_t n = 0;
+  size_t bytes = 0;  // the block's size (at least n * sizeof(T))
+  int device = 0;
   DevBuf() = default;
   DevBuf(const DevBuf &) = delete;
   DevBuf &operator=(const DevBuf &) = delete;
   ~DevBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) device_free(p, bytes, device);
     p = nullptr;
     n = 0;
+    bytes = 0;
   }
   void ensure(size_t count) {
     if (n >= count && p) return;
     release();
     if (count == 0) return;
-    hipError_t e = hipMalloc(&p, count * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      if (e == hipErrorOutOfMemory) throw std::bad_alloc();
-      throw hip_error(std::string("hipMalloc failed: ") + hipGetErrorString(e));
-    }
+    p = static_cast<T *>(device_alloc(count * sizeof(T), &bytes, &device));
     n = count;
+  }
+  void swap(DevBuf &o) {
+    std::swap(p, o.p);
+    std::swap(n, o.n);
+    std::swap(bytes, o.bytes);
+    std::swap(device, o.device);
   }
 };
 

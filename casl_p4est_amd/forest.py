@@ -25,6 +25,26 @@ from .levelset import LevelSet as LevelSetFunction
 P4EST_MAXLEVEL = {2: 30, 3: 19}
 
 
+def device_cache_set_limit(bytes_per_device):
+    """Keep device blocks the library releases, up to bytes_per_device per
+    GPU, for reuse by its later allocations of the same size
+    (casl_device_cache_set_limit); 0, the default, hands them back at once.
+    For regrid loops: build a grid on the device, drop the old one, repeat."""
+    L.call("casl_device_cache_set_limit", ctypes.c_size_t(int(bytes_per_device)))
+
+
+def device_cache_trim():
+    """Free every block the device cache holds (casl_device_cache_trim)."""
+    L.call("casl_device_cache_trim")
+
+
+def device_cache_held(device=0):
+    """Bytes the device cache holds on one GPU (casl_device_cache_held)."""
+    b = ctypes.c_size_t()
+    L.call("casl_device_cache_held", int(device), ctypes.byref(b))
+    return b.value
+
+
 class Forest:
     """A forest of quadtrees (dim=2) or octrees (dim=3) on [box_min, box_max]:
     n_trees[a] trees along axis a (default one tree), periodic[a] wrapping
@@ -308,6 +328,12 @@ class NodeNeighbors:
                 self.init_neighbors()
             self._device = DeviceStencil(self.tables)
         return self._device
+
+    def release_device(self):
+        """Drop this object's reference to the device copy of the tables; its
+        device memory is freed once no other reference holds it, and the next
+        device() uploads the tables again."""
+        self._device = None
 
     def second_derivatives_central(self, f, fdd=None):
         """fdd[a][n] = d^2f/dx_a^2 at node n, on the GPU (torch float64 tensors)."""

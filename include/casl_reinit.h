@@ -30,6 +30,7 @@
 #ifndef CASL_REINIT_H
 #define CASL_REINIT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -235,6 +236,24 @@ int casl_band_linf(casl_stencil_t *st, const double *a, const double *b,
 int casl_profile_enable(int enable);
 int casl_profile_reset(void);
 int casl_profile_query(const char *kernel, double *total_ms, int64_t *launches);
+
+/* ------------------------------------------------- device block cache
+ * No counterpart in the reference (its grids live in host memory).  Device
+ * memory the library releases (a destroyed grid's or stencil's tables, the
+ * temporaries of a device grid build or tiling) goes back to the driver at
+ * once by default.  With a limit above 0, released blocks up to that many
+ * bytes per device are kept and reused by later allocations of the library
+ * of the same size (at most 25% larger): a regrid loop that builds a grid on
+ * the device, drops the previous one and builds again then stops waiting on
+ * the driver's hipMalloc of recently freed memory (seconds at hundreds of
+ * GB).  A block enters the cache after a device-wide synchronization, as
+ * hipFree would wait.  Cached memory is unavailable to other allocators
+ * (torch's included) until casl_device_cache_trim frees it; an allocation of
+ * the library that fails trims its device's cache and tries once more.
+ * Lowering the limit below what a device holds trims that device. */
+int casl_device_cache_set_limit(size_t bytes_per_device);
+int casl_device_cache_trim(void);
+int casl_device_cache_held(int device, size_t *bytes);
 
 /* ------------------------------------------------- batched small trees
  * Many independent trees reinitialized by one persistent kernel.  The trees'

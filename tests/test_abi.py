@@ -42,3 +42,15 @@ def test_error_codes_on_bad_arguments():
     assert b"dim" in L.lib().casl_last_error()
     rc = L.lib().casl_reinit(None, None, 2, 2, 1, 1.0, -1.0, 0, None)
     assert rc == L.CASL_ERR_INVALID_ARGUMENT
+
+
+def test_device_cache_bookkeeping():
+    """The cache's limit, held-bytes and trim calls; with nothing cached they
+    touch no device, so they run here without a GPU."""
+    import casl_p4est_amd as C
+    assert C.device_cache_held(0) == 0
+    C.device_cache_set_limit(1 << 30)
+    assert C.device_cache_held(0) == 0
+    C.device_cache_trim()
+    C.device_cache_set_limit(0)
+    assert L.lib().casl_device_cache_held(0, None) == L.CASL_ERR_INVALID_ARGUMENT

@@ -186,3 +186,42 @@ def test_device_grid_workloads(cuda, name, lmax):
     assert info["tiled"]
     ok, why = ds.tile_check(tab)
     assert ok, why
+
+
+def test_device_cache_regrid_loop(cuda):
+    """A regrid loop with the device block cache on: grids built from blocks
+    that dropped grids released (old contents and all) equal a grid built
+    with the cache off, table for table, and reinitialize to the same bits on
+    both stage paths; the cache holds the released blocks until trimmed."""
+    import torch
+    dev = torch.cuda.current_device()
+    phi = ls.Sphere([0.45, 0.5, 0.55], 0.3)
+    f, tab = host_grid(3, phi, 2, 7, 1.2, 0.0)
+    phi0 = torch.from_numpy(f.sample(phi)).to(cuda)
+    ref = {}
+    g = C.DeviceGrid(3, phi, 2, 7, lip=1.2)
+    for path in ("tiled", "two_pass"):
+        g.device().set_path(path)
+        a = phi0.clone()
+        g.device().reinit(a, 2, 2, 5, C.DBL_MAX, -C.DBL_MAX, 1)
+        ref[path] = a
+    del g
+    assert C.device_cache_held(dev) == 0
+    C.device_cache_set_limit(16 << 30)
+    try:
+        for it in range(3):
+            say(f"[cache] regrid {it}, held {C.device_cache_held(dev)} bytes")
+            g = C.DeviceGrid(3, phi, 2, 7, lip=1.2)
+            assert_same_grid(f, tab, g)
+            for path in ("tiled", "two_pass"):
+                g.device().set_path(path)
+                b = phi0.clone()
+                g.device().reinit(b, 2, 2, 5, C.DBL_MAX, -C.DBL_MAX, 1)
+                assert torch.equal(ref[path], b), (it, path)
+            del g
+            assert C.device_cache_held(dev) > 0
+        C.device_cache_trim()
+        assert C.device_cache_held(dev) == 0
+    finally:
+        C.device_cache_set_limit(0)
+        C.device_cache_trim()

@@ -515,6 +515,55 @@ def cpu_baseline(W, budget_s, threads=0):
                        f"{threads} OpenMP thread{'s' if threads > 1 else ''}, {t1:.1f} s")
 
 
+def run_grid_only(args, torch):
+    """Device grid construction alone for a tree workload: refinement, node
+    numbering and construct_neighbors, then the device stencil with its tiled
+    encoding built on the device.  With the device block cache on (off with
+    --no-cache): one cold build, its blocks from hipMalloc, then a regrid loop
+    of `steps` builds, each dropping the previous grid.  No host grid is
+    built: the GPU tests compare the device grid with the host one."""
+    import casl_p4est_amd as C
+    from casl_p4est_amd import workloads
+    cfg = dict(workloads.TREES[args.config])
+    if args.lmax:
+        cfg["lmax"] = args.lmax
+    shape = workloads.tree_shape(args.config)
+    torch.cuda.init()
+
+    def once(label):
+        t0 = time.perf_counter()
+        g = C.DeviceGrid(cfg["dim"], shape, cfg["lmin"], cfg["lmax"], lip=2.4)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        info = g.device().tile_info()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        sizes = g.sizes()
+        log(f"[bench] {label}: grid {t1 - t0:.3f} s, stencil and tiling {t2 - t1:.3f} s, "
+            f"{sizes[1]} nodes, {info['n_tiles']} tiles")
+        del g
+        return t1 - t0, t2 - t1, sizes, info
+
+    cache = not args.no_cache
+    if cache:
+        C.device_cache_set_limit(int(0.9 * torch.cuda.get_device_properties(0).total_memory))
+    try:
+        # the cold build's blocks come from hipMalloc (the cache starts empty)
+        cold = once("cold build")
+        runs = [once(f"regrid {k}") for k in range(max(1, args.steps))]
+    finally:
+        C.device_cache_set_limit(0)
+    for r in runs:
+        if r[2] != cold[2] or r[3] != cold[3]:
+            raise RuntimeError("repeated device builds differ")
+    return {"metric": "device grid construction s (refine + nodes + construct_neighbors)", "value": min(r[0] for r in runs),
+            "unit": "s", "higher_is_better": False, "config": {"workload": cfg["desc"], "leaves": cold[2][0],
+                                                             "nodes": cold[2][1]},
+            "stencil_and_tiling_s": min(r[1] for r in runs), "cold": {"grid_s": cold[0], "stencil_and_tiling_s": cold[1]},
+            "regrid_loop": [{"grid_s": r[0], "stencil_and_tiling_s": r[1]} for r in runs], "device_block_cache": cache,
+            "tiles": cold[3]}
+
+
 def extras(args, W, lsr, phi0, out_mixed, dev, torch, C):
     """advect_in_normal_direction, grid construction on the device and (3D)
     network curvature on the benchmark's grid."""
@@ -643,6 +692,9 @@ def main():
     ap.add_argument("--lmax", type=int, default=0, help="reduce max level (quick runs only)")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid-only", action="store_true",
+                    help="tree configs: time the device grid construction and tiling alone (a regrid loop)")
+    ap.add_argument("--no-cache", action="store_true", help="--grid-only: the regrid loop without the device block cache")
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="threads of the CPU baseline (default: 1 for star2d, all cores otherwise)")
     ap.add_argument("--path", default="auto", choices=["auto", "tiled", "two_pass"],
@@ -667,6 +719,9 @@ def main():
     if args.config == "curvature3d":
         args.epochs = args.epochs or 100
         print(json.dumps(run_curvature3d(args, torch)), flush=True)
+        return
+    if args.grid_only:
+        print(json.dumps(run_grid_only(args, torch)), flush=True)
         return
     dev = torch.device("cuda:0")
     W = build_workload(args.config, args.lmax)

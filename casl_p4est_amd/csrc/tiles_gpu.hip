@@ -120,32 +120,66 @@ __global__ void k_check_convert(DTab<D> t, float *__restrict__ dist, int *__rest
 }
 
 // ---------------------------------------------------------------- scales
+// Blocks of the two scale reductions: each thread strides over the nodes, the
+// block reduces through wave shuffles and LDS, and one atomic per block
+// reaches memory (one per node serialized on the few target words).
+constexpr unsigned REDUCE_BLOCKS = 2048;
+inline unsigned reduce_grid(int64_t n) { return std::max(1u, std::min(REDUCE_BLOCKS, grid_for(n))); }
+
+template <class T>
+__device__ __forceinline__ T block_max(T v) {
+  __shared__ T part[BLOCK / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const T w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  __syncthreads();  // part[] may still be read by a previous call
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 1; k < BLOCK / 64; ++k) v = part[k] > v ? part[k] : v;
+  return v;  // the block's maximum in thread 0
+}
+
 // max arm cell per axis, out[D] (non-negative doubles order like their bit
 // patterns)
 template <int D>
-__global__ void k_arm_max(DTab<D> t, unsigned long long *__restrict__ out, int *__restrict__ bad) {
-  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (n >= t.N) return;
+__global__ void __launch_bounds__(BLOCK) k_arm_max(DTab<D> t, unsigned long long *__restrict__ out,
+                                                   int *__restrict__ bad) {
+  double m[D];
+#pragma unroll
+  for (int a = 0; a < D; ++a) m[a] = 0.0;
+  for (int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < t.N; n += int64_t(gridDim.x) * blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+      for (int f = 2 * a; f < 2 * a + 2; ++f) {
+        const double q = t.arm_cell(f, n);
+        if (!(q > 0)) flag(bad, BAD_SCALE);
+        m[a] = fmax(m[a], q);
+      }
+  }
+#pragma unroll
   for (int a = 0; a < D; ++a) {
-    double m = 0.0;
-    for (int f = 2 * a; f < 2 * a + 2; ++f) {
-      const double q = t.arm_cell(f, n);
-      if (!(q > 0)) flag(bad, BAD_SCALE);
-      m = fmax(m, q);
-    }
-    atomicMax(out + a, (unsigned long long)__double_as_longlong(m));
+    const unsigned long long b = block_max((unsigned long long)__double_as_longlong(m[a]));
+    if (threadIdx.x == 0) atomicMax(out + a, b);
   }
 }
 
-// level_of of every arm cell (as the host builder: nearest integer of
-// log2(base / q), then base * 2^-l must be q)
+// level_of of every arm cell.  The host builder takes l = the nearest
+// integer of log2(base / q) and requires base * 2^-l == q with 0 <= l <= 15;
+// that accepts exactly the q with base's significand and a binary exponent l
+// below base's, and returns that l, so the exponents are compared here
+// without the logarithm.
 __device__ __forceinline__ int level_of(double base, double q, int *bad) {
   if (!(q > 0)) {
     flag(bad, BAD_SCALE);
     return 0;
   }
-  const int l = int(lround(log2(base / q)));
-  if (l < 0 || l > 15 || base * ldexp(1.0, -l) != q) {
+  int eb = 0, eq = 0;
+  const double mb = frexp(base, &eb), mq = frexp(q, &eq);
+  const int l = eb - eq;
+  if (mb != mq || l < 0 || l > 15) {
     flag(bad, BAD_SCALE);
     return 0;
   }
@@ -153,12 +187,13 @@ __device__ __forceinline__ int level_of(double base, double q, int *bad) {
 }
 
 template <int D>
-__global__ void k_level_max(DTab<D> t, Scales sc, int *__restrict__ out, int *__restrict__ bad) {
-  const int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (n >= t.N) return;
+__global__ void __launch_bounds__(BLOCK) k_level_max(DTab<D> t, Scales sc, int *__restrict__ out,
+                                                     int *__restrict__ bad) {
   int m = 0;
-  for (int f = 0; f < DTab<D>::F; ++f) m = max(m, level_of(sc.base[f >> 1], t.arm_cell(f, n), bad));
-  atomicMax(out, m);
+  for (int64_t n = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; n < t.N; n += int64_t(gridDim.x) * blockDim.x)
+    for (int f = 0; f < DTab<D>::F; ++f) m = max(m, level_of(sc.base[f >> 1], t.arm_cell(f, n), bad));
+  m = block_max(m);
+  if (threadIdx.x == 0) atomicMax(out, m);
 }
 
 // ---------------------------------------------------------------- halos
@@ -274,37 +309,53 @@ struct TileView {
   Scales sc;
 };
 
-// the tile of global record position p and the position i within it
-__device__ __forceinline__ void position(const TileView &v, int64_t p, int &tl, int &i) {
-  int lo = 0, hi = v.nt;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (v.rpos[mid] <= p) lo = mid;
-    else hi = mid;
+// A tile handled by one block: its halo node ids, H1 then H2 (each in
+// ascending order), staged in LDS, and the LDS slot of a node of the tile.
+template <int D>
+struct TileLocal {
+  int32_t s = 0, e = 0;  // the tile's node range
+  int n1 = 0, n2 = 0;    // H1 and H2 entries
+  const int32_t *h = nullptr;
+  // LDS slot of node g in T u H1 u H2, -1 if not there
+  __device__ int32_t operator()(int32_t g) const {
+    if (g >= s && g < e) return g - s;
+    int p = lower(0, n1, g);
+    if (p < n1 && h[p] == g) return (e - s) + p;
+    p = lower(n1, n1 + n2, g);
+    if (p < n1 + n2 && h[p] == g) return (e - s) + p;
+    return -1;
   }
-  tl = lo;
-  i = int(p - v.rpos[lo]);
-}
+  // node of record position i (the tile's nodes, then its H1 nodes)
+  __device__ int64_t node(int i) const { return i < e - s ? int64_t(s) + i : int64_t(h[i - (e - s)]); }
+  __device__ int lower(int b, int end, int32_t g) const {
+    while (b < end) {
+      const int m = (b + end) >> 1;
+      if (h[m] < g) b = m + 1;
+      else end = m;
+    }
+    return b;
+  }
+};
 
-// node of position i of tile tl
-__device__ __forceinline__ int64_t node_at(const TileView &v, int tl, int i) {
-  const int nT = v.ts[tl + 1] - v.ts[tl];
-  if (i < nT) return int64_t(v.ts[tl]) + i;
-  return int64_t(uint32_t(v.h1[v.h1seg[tl] + (i - nT)]));
-}
-
-// local index (LDS slot) of node g in tile tl, -1 if not in T u H1 u H2
-__device__ __forceinline__ int32_t local_of(const TileView &v, int tl, int32_t g) {
-  const int32_t s = v.ts[tl], e = v.ts[tl + 1];
-  if (g >= s && g < e) return g - s;
-  const uint64_t key = hkey(tl, g);
-  const int64_t b1 = v.h1seg[tl], e1 = v.h1seg[tl + 1];
-  const int64_t p1 = find_key(v.h1, b1, e1, key);
-  if (p1 < e1 && v.h1[p1] == key) return int32_t(e - s) + int32_t(p1 - b1);
-  const int64_t b2 = v.h2seg[tl], e2 = v.h2seg[tl + 1];
-  const int64_t p2 = find_key(v.h2, b2, e2, key);
-  if (p2 < e2 && v.h2[p2] == key) return int32_t(e - s) + int32_t(e1 - b1) + int32_t(p2 - b2);
-  return -1;
+// Stages tile tl's halo ids into hl (every thread of the block calls it);
+// false (and an error bit) if they exceed the LDS array, which the tile
+// limits rule out.
+template <int D>
+__device__ __forceinline__ bool stage_tile(const TileView &v, int tl, int32_t *hl, TileLocal<D> &loc, int *bad) {
+  loc.s = v.ts[tl];
+  loc.e = v.ts[tl + 1];
+  const int64_t b1 = v.h1seg[tl], b2 = v.h2seg[tl];
+  loc.n1 = int(v.h1seg[tl + 1] - b1);
+  loc.n2 = int(v.h2seg[tl + 1] - b2);
+  loc.h = hl;
+  if (loc.n1 + loc.n2 > TileLimits<D>::LMAX) {
+    if (threadIdx.x == 0) flag(bad, BAD_BOUNDS);
+    return false;
+  }
+  for (int j = int(threadIdx.x); j < loc.n1 + loc.n2; j += int(blockDim.x))
+    hl[j] = int32_t(uint32_t(j < loc.n1 ? v.h1[b1 + j] : v.h2[b2 + (j - loc.n1)]));
+  __syncthreads();
+  return true;
 }
 
 __device__ __forceinline__ void transverse(int D, int a, int &t1, int &t2) {
@@ -321,8 +372,8 @@ __device__ __forceinline__ void transverse(int D, int a, int &t1, int &t2) {
 // nonzero weight, levels, walls, hanging faces and their extension words.
 // Returns the number of hanging faces (-1 when the record cannot be encoded).
 template <int D>
-__device__ int encode(const DTab<D> &t, const TileView &v, int tl, int64_t n, TileRec &r, uint16_t *e,
-                      int *bad) {
+__device__ int encode(const DTab<D> &t, const TileView &v, const TileLocal<D> &loc, int64_t n, TileRec &r,
+                      uint16_t *e, int *bad) {
   constexpr int F = DTab<D>::F, C = DTab<D>::C;
   constexpr int NH = HangExt<D>::NH, WORDS = HangExt<D>::WORDS, SW = HangExt<D>::SW;
   uint32_t geo = 0;
@@ -342,7 +393,7 @@ __device__ int encode(const DTab<D> &t, const TileView &v, int tl, int64_t n, Ti
       flag(bad, BAD_ENCODE);
       return -1;
     }
-    const int32_t l = local_of(v, tl, t.nb(f, first, n));
+    const int32_t l = loc(t.nb(f, first, n));
     if (l < 0 || l > 65535) {
       flag(bad, BAD_ENCODE);
       return -1;
@@ -372,7 +423,7 @@ __device__ int encode(const DTab<D> &t, const TileView &v, int tl, int64_t n, Ti
         if (w[c] != 0.0 && first < 0) first = c;
       uint16_t *fe = e + 1 + q * SW;
       for (int c = 0; c < C; ++c) {
-        const int32_t l = local_of(v, tl, t.nb(f, w[c] != 0.0 ? c : first, n));
+        const int32_t l = loc(t.nb(f, w[c] != 0.0 ? c : first, n));
         if (l < 0 || l > 65535) {
           flag(bad, BAD_ENCODE);
           return -1;
@@ -397,9 +448,11 @@ __device__ int encode(const DTab<D> &t, const TileView &v, int tl, int64_t n, Ti
 }
 
 // The host builder's verify(): the decoded record reproduces the tables
-// (distances bit-exact, nonzero-weight corners equal).
+// (distances bit-exact, one nonzero-weight corner on a face without a
+// hanging arm).  Its slots are what encode() looked up for the same nodes
+// just before, so they are not looked up again.
 template <int D>
-__device__ void verify(const DTab<D> &t, const TileView &v, int tl, int64_t n, const TileRec &r, const uint16_t *e,
+__device__ void verify(const DTab<D> &t, const TileView &v, int64_t n, const TileRec &r, const uint16_t *e,
                        int *bad) {
   constexpr int F = DTab<D>::F, C = DTab<D>::C, SW = HangExt<D>::SW;
   const double *base = v.sc.base;
@@ -421,17 +474,11 @@ __device__ void verify(const DTab<D> &t, const TileView &v, int tl, int64_t n, c
     const int q = (f == hf[0]) ? 0 : (f == hf[1]) ? 1 : -1;
     if (q < 0) {
       int nz = 0;
-      for (int c = 0; c < C; ++c)
-        if (w[c] != 0.0) {
-          ++nz;
-          if (local_of(v, tl, t.nb(f, c, n)) != r.idx[f]) flag(bad, BAD_VERIFY);
-        }
+      for (int c = 0; c < C; ++c) nz += (w[c] != 0.0) ? 1 : 0;
       if (nz != 1) flag(bad, BAD_VERIFY);
       continue;
     }
     const uint16_t *fe = e + 1 + q * SW;
-    for (int c = 0; c < C; ++c)
-      if (w[c] != 0.0 && local_of(v, tl, t.nb(f, c, n)) != fe[c]) flag(bad, BAD_VERIFY);
     // the arm cell's transverse extents, at the face's level
     int t1, t2;
     transverse(D, f >> 1, t1, t2);
@@ -464,21 +511,24 @@ __device__ __forceinline__ int group_of(const TileRec &r, int nh, bool h1) {
   return g + (h1 ? 4 : 0);
 }
 
-// sort key (tile, group) of every record position; the value is the position
+// sort key (tile, group) of every record position; the value is the
+// position.  One block per tile.
 template <int D>
-__global__ void k_classify(DTab<D> t, TileView v, int64_t R, uint64_t *__restrict__ keys, int64_t *__restrict__ vals,
-                           int *__restrict__ bad) {
-  const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (p >= R) return;
-  int tl, i;
-  position(v, p, tl, i);
-  const int64_t n = node_at(v, tl, i);
-  TileRec r;
-  uint16_t e[HangExt<D>::WORDS];
-  const int nh = encode<D>(t, v, tl, n, r, e, bad);
-  const int nT = v.ts[tl + 1] - v.ts[tl];
-  keys[p] = (uint64_t(uint32_t(tl)) << 3) | uint32_t(group_of<D>(r, nh < 0 ? 0 : nh, i >= nT));
-  vals[p] = p;
+__global__ void __launch_bounds__(BLOCK) k_classify(DTab<D> t, TileView v, uint64_t *__restrict__ keys,
+                                                    int64_t *__restrict__ vals, int *__restrict__ bad) {
+  __shared__ int32_t hl[TileLimits<D>::LMAX];
+  const int tl = int(blockIdx.x);
+  TileLocal<D> loc;
+  if (!stage_tile<D>(v, tl, hl, loc, bad)) return;
+  const int nT = loc.e - loc.s;
+  const int64_t p0 = v.rpos[tl];
+  for (int i = int(threadIdx.x); i < nT + loc.n1; i += BLOCK) {
+    TileRec r;
+    uint16_t e[HangExt<D>::WORDS];
+    const int nh = encode<D>(t, v, loc, loc.node(i), r, e, bad);
+    keys[p0 + i] = (uint64_t(uint32_t(tl)) << 3) | uint32_t(group_of<D>(r, nh < 0 ? 0 : nh, i >= nT));
+    vals[p0 + i] = p0 + i;
+  }
 }
 
 // per tile: where each group starts in the sorted order (gs[t * 9 + g])
@@ -531,34 +581,45 @@ struct TileOut {
 };
 
 // every record, encoded again, checked and written at its place in the
-// host builder's order
+// host builder's order.  One block per tile: its records are the sorted
+// positions gs[tl * 9] .. gs[tl * 9 + 8].
 template <int D>
-__global__ void k_emit(DTab<D> t, TileView v, const int64_t *__restrict__ sorted_p, const uint64_t *__restrict__ keys,
-                       const int64_t *__restrict__ gs, int64_t R, TileOut o, int *__restrict__ bad) {
-  const int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (k >= R) return;
-  const int64_t p = sorted_p[k];
-  int tl, i;
-  position(v, p, tl, i);
-  const int g = int(keys[k] & 7u);
-  const int64_t n = node_at(v, tl, i);
-  TileRec r;
-  uint16_t e[HangExt<D>::WORDS];
-  const int nh = encode<D>(t, v, tl, n, r, e, bad);
-  if (nh < 0) return;
-  verify<D>(t, v, tl, n, r, e, bad);
+__global__ void __launch_bounds__(BLOCK) k_emit(DTab<D> t, TileView v, const int64_t *__restrict__ sorted_p,
+                                                const uint64_t *__restrict__ keys, const int64_t *__restrict__ gs,
+                                                TileOut o, int *__restrict__ bad) {
+  __shared__ int32_t hl[TileLimits<D>::LMAX];
+  const int tl = int(blockIdx.x);
+  TileLocal<D> loc;
+  if (!stage_tile<D>(v, tl, hl, loc, bad)) return;
   const int64_t *G = gs + int64_t(tl) * 9;
-  if (g == 0 || g == 4) {
-    const int64_t u = o.urs[tl] + (g == 0 ? k - G[0] : (G[1] - G[0]) + (k - G[4]));
-    uni_pack<D>(r.idx, uint32_t(i), r.geo & 15u, o.urec + u * UniRec<D>::WORDS);
-    return;
-  }
-  const int64_t q = o.rs[tl] + (g < 4 ? k - G[1] : (G[4] - G[1]) + (k - G[5]));
-  o.rec[q] = r;
-  o.rec_slot[q] = uint16_t(i);
-  if (nh > 0) {
-    const int64_t x = o.xs[tl] + (g < 4 ? k - G[2] : (G[4] - G[2]) + (k - G[6]));
-    for (int w = 0; w < HangExt<D>::WORDS; ++w) o.ext[x * HangExt<D>::WORDS + w] = e[w];
+  const int64_t p0 = v.rpos[tl];
+  const int nr = (loc.e - loc.s) + loc.n1;
+  for (int64_t k = G[0] + threadIdx.x; k < G[8]; k += BLOCK) {
+    const int64_t ip = sorted_p[k] - p0;
+    if (ip < 0 || ip >= nr) {
+      flag(bad, BAD_BOUNDS);
+      continue;
+    }
+    const int i = int(ip);
+    const int g = int(keys[k] & 7u);
+    const int64_t n = loc.node(i);
+    TileRec r;
+    uint16_t e[HangExt<D>::WORDS];
+    const int nh = encode<D>(t, v, loc, n, r, e, bad);
+    if (nh < 0) continue;
+    verify<D>(t, v, n, r, e, bad);
+    if (g == 0 || g == 4) {
+      const int64_t u = o.urs[tl] + (g == 0 ? k - G[0] : (G[1] - G[0]) + (k - G[4]));
+      uni_pack<D>(r.idx, uint32_t(i), r.geo & 15u, o.urec + u * UniRec<D>::WORDS);
+      continue;
+    }
+    const int64_t q = o.rs[tl] + (g < 4 ? k - G[1] : (G[4] - G[1]) + (k - G[5]));
+    o.rec[q] = r;
+    o.rec_slot[q] = uint16_t(i);
+    if (nh > 0) {
+      const int64_t x = o.xs[tl] + (g < 4 ? k - G[2] : (G[4] - G[2]) + (k - G[6]));
+      for (int w = 0; w < HangExt<D>::WORDS; ++w) o.ext[x * HangExt<D>::WORDS + w] = e[w];
+    }
   }
 }
 
@@ -712,7 +773,7 @@ void build(const DTab<D> &tab, casl_stencil *st, hipStream_t s) {
   lmx.ensure(1);
   HIP_CHECK(hipMemsetAsync(amax.p, 0, D * sizeof(unsigned long long), s));
   HIP_CHECK(hipMemsetAsync(lmx.p, 0, sizeof(int), s));
-  hipLaunchKernelGGL(k_arm_max<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, tab, amax.p, bad.p);
+  hipLaunchKernelGGL(k_arm_max<D>, dim3(reduce_grid(N)), dim3(BLOCK), 0, s, tab, amax.p, bad.p);
   HIP_CHECK(hipGetLastError());
   // as the host builder: one scale per axis (axes past D keep the box edge 1)
   unsigned long long bits[3] = {0, 0, 0};
@@ -721,7 +782,7 @@ void build(const DTab<D> &tab, casl_stencil *st, hipStream_t s) {
   Scales sc;
   for (int a = 0; a < 3; ++a) sc.base[a] = 1.0;
   for (int a = 0; a < D; ++a) std::memcpy(&sc.base[a], &bits[a], sizeof(double));
-  hipLaunchKernelGGL(k_level_max<D>, dim3(grid_for(N)), dim3(BLOCK), 0, s, tab, sc, lmx.p, bad.p);
+  hipLaunchKernelGGL(k_level_max<D>, dim3(reduce_grid(N)), dim3(BLOCK), 0, s, tab, sc, lmx.p, bad.p);
   HIP_CHECK(hipGetLastError());
   sc.lmax = read_back(lmx.p, s);
   for (int a = 0; a < 3; ++a) sc.unit[a] = sc.base[a] * std::ldexp(1.0, -sc.lmax);
@@ -824,7 +885,7 @@ void build(const DTab<D> &tab, casl_stencil *st, hipStream_t s) {
   rvals.ensure(size_t(R));
   rkeys_sorted.ensure(size_t(R));
   rvals_sorted.ensure(size_t(R));
-  hipLaunchKernelGGL(k_classify<D>, dim3(grid_for(R)), dim3(BLOCK), 0, s, tab, v, R, rkeys.p, rvals.p, bad.p);
+  hipLaunchKernelGGL(k_classify<D>, dim3(unsigned(nt)), dim3(BLOCK), 0, s, tab, v, rkeys.p, rvals.p, bad.p);
   HIP_CHECK(hipGetLastError());
   b = read_back(bad.p, s);
   if (b) throw std::runtime_error(bad_reason(b));
@@ -860,7 +921,7 @@ void build(const DTab<D> &tab, casl_stencil *st, hipStream_t s) {
   st->t_halo.ensure(size_t(std::max<int64_t>(n_halo, 1)));
   TileOut o{reinterpret_cast<TileRec *>(st->t_rec.p), st->t_rec_slot.p, st->t_urec.p, st->t_ext.p, st->t_halo.p,
             urs.p, rs.p, xs.p, hs.p};
-  hipLaunchKernelGGL(k_emit<D>, dim3(grid_for(R)), dim3(BLOCK), 0, s, tab, v, rvals_sorted.p, rkeys_sorted.p, gs.p, R,
+  hipLaunchKernelGGL(k_emit<D>, dim3(unsigned(nt)), dim3(BLOCK), 0, s, tab, v, rvals_sorted.p, rkeys_sorted.p, gs.p,
                      o, bad.p);
   HIP_CHECK(hipGetLastError());
   if (nh1 > 0)

@@ -29,9 +29,11 @@ struct BlockCache {
   std::map<int, std::multimap<size_t, void *>> blocks;  // device -> size -> block
 };
 
+// never destroyed: blocks released during static destruction at exit still
+// find it (the driver reclaims whatever it holds then)
 BlockCache &block_cache() {
-  static BlockCache c;
-  return c;
+  static BlockCache *c = new BlockCache;
+  return *c;
 }
 
 // allocation sizes: 256-byte granules, 2 MiB ones from 1 MiB up (so that the

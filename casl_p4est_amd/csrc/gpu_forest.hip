@@ -572,8 +572,11 @@ void build_grid(casl_device_grid *g, int min_lvl, int max_lvl, double lip, doubl
     hipLaunchKernelGGL(k_corner_keys<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, q.p, L, ka.p);
     HIP_CHECK(hipGetLastError());
     const int end_bit = D * GG::MAXLEVEL;
-    hipcub::DoubleBuffer<uint64_t> db(ka.p, kb.p);
-    cub_call(s, [&](void *t, size_t &b) { return hipcub::DeviceRadixSort::SortKeys(t, b, db, K, 0, end_bit, s); });
+    hipcub::DoubleBuffer<uint64_t> db;
+    cub_call(s, [&](void *t, size_t &b) {
+      db = hipcub::DoubleBuffer<uint64_t>(ka.p, kb.p);  // the size query and the sort both start from ka
+      return hipcub::DeviceRadixSort::SortKeys(t, b, db, K, 0, end_bit, s);
+    });
     uint64_t *sorted = db.Current(), *nodes = db.Alternate();
     cub_call(s, [&](void *t, size_t &b) { return hipcub::DeviceSelect::Unique(t, b, sorted, nodes, nsel.p, K, s); });
     HIP_CHECK(hipMemcpyAsync(&N, nsel.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));

@@ -571,11 +571,15 @@ void build_grid(casl_device_grid *g, int min_lvl, int max_lvl, double lip, doubl
     nsel.ensure(1);
     hipLaunchKernelGGL(k_corner_keys<D>, dim3(grid_for(L)), dim3(BLOCK), 0, s, q.p, L, ka.p);
     HIP_CHECK(hipGetLastError());
-    const int end_bit = D * GG::MAXLEVEL;
+    // Every corner coordinate is a multiple of 2^z, z = MAXLEVEL - finest,
+    // or ROOT_LEN - 1 (clamped): its low z bits are all 0 or all 1, so bit
+    // z - 1 fixes the z - 1 bits below it.  Keys that agree from bit
+    // D (z - 1) up are equal, and the sort can start there.
+    const int end_bit = D * GG::MAXLEVEL, begin_bit = D * (GG::MAXLEVEL - finest - 1);
     hipcub::DoubleBuffer<uint64_t> db;
     cub_call(s, [&](void *t, size_t &b) {
       db = hipcub::DoubleBuffer<uint64_t>(ka.p, kb.p);  // the size query and the sort both start from ka
-      return hipcub::DeviceRadixSort::SortKeys(t, b, db, K, 0, end_bit, s);
+      return hipcub::DeviceRadixSort::SortKeys(t, b, db, K, begin_bit, end_bit, s);
     });
     uint64_t *sorted = db.Current(), *nodes = db.Alternate();
     cub_call(s, [&](void *t, size_t &b) { return hipcub::DeviceSelect::Unique(t, b, sorted, nodes, nsel.p, K, s); });

@@ -143,15 +143,21 @@ __global__ void k_batch_copy(const double *__restrict__ phi, double *__restrict_
 }
 
 // Register caps of k_batch_tiles (waves per SIMD): its 256- and 512-thread
-// blocks run four or two tile nodes per thread, which need up to 128 VGPRs
-// in FP32 and 168 in FP64.
+// blocks run several tile nodes per thread, which need up to 128 VGPRs in
+// FP32 and 168 in FP64; the 1024-thread blocks of 2D trees (two nodes of a
+// 2048-node tile per thread) keep FP32 at 64 VGPRs, two blocks per compute
+// unit, and FP64 at 128.
 constexpr int BATCH_WAVES_F32 = 4, BATCH_WAVES_F64 = 3;
+template <class R, int BLOCK>
+constexpr int batch_waves() {
+  if (BLOCK == 1024) return sizeof(R) == 4 ? 8 : 4;
+  return sizeof(R) == 4 ? BATCH_WAVES_F32 : BATCH_WAVES_F64;
+}
 
 // Persistent kernel over trees whose pseudo-time stages run tile by tile in
 // LDS; every tree's result is its single-tree tiled result.
 template <int D, class R, int BLOCK>
-__global__ void __launch_bounds__(BLOCK)
-    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? BATCH_WAVES_F32 : BATCH_WAVES_F64)))
+__global__ void __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(batch_waves<R, BLOCK>())))
     k_batch_tiles(BatchTileArgs<R> A) {
   __shared__ R phis[TileLimits<D>::LMAX];
   __shared__ R dds[D * TileLimits<D>::DMAX];
@@ -217,14 +223,16 @@ KernelFn<D, R> pick_kernel(const Schedule &s) {
 // block sizes of the tiled kernel: a tile's nodes per thread stay small
 template <int D>
 constexpr bool tile_block_ok(int block) {
-  return block == 512 || (D == 2 && block == 256);
+  return block == 512 || (D == 2 && (block == 256 || block == 1024));
 }
 
 template <int D, class R>
 TileKernelFn<D, R> pick_tile_kernel(int block) {
   if (block == 512) return &k_batch_tiles<D, R, 512>;
-  if constexpr (D == 2)
+  if constexpr (D == 2) {
     if (block == 256) return &k_batch_tiles<D, R, 256>;
+    if (block == 1024) return &k_batch_tiles<D, R, 1024>;
+  }
   throw std::invalid_argument("unsupported tiled batch block size");
 }
 
@@ -642,7 +650,7 @@ int casl_batch_autotune(casl_batch_t *b, const double *phi, int n_iter, int prec
     for (int bt : kBlocks)
       for (int npt : kNpts) cands.push_back(Schedule{bt, npt});
     if (b->tiled) {
-      for (int bt : {256, 512})
+      for (int bt : {256, 512, 1024})
         if (b->st.dim == 2 ? tile_block_ok<2>(bt) : tile_block_ok<3>(bt)) cands.push_back(Schedule{bt, 0});
     }
     try {

@@ -61,13 +61,14 @@ constexpr uint32_t GEO_IRREGULAR = 1u << 30;
 //   3D (3 words): word k = idx[2k] (bits 0-12) | idx[2k+1] (13-25) | x_k (26-31)
 //                 with x_0, x_1 the low and high 6 bits of the LDS slot and
 //                 x_2 the arm level (4 bits)
-//   2D (2 words): word 0 = idx[0] (0-10) | idx[1] (11-21) | slot bits 0-9 (22-31)
-//                 word 1 = idx[2] (0-10) | idx[3] (11-21) | slot bit 10 (22) | level (23-26)
+//   2D (2 words): word 0 = idx[0] (0-11) | idx[1] (12-23) | slot bits 0-7 (24-31)
+//                 word 1 = idx[2] (0-11) | idx[3] (12-23) | slot bits 8-11 (24-27) | level (28-31)
 template <int D>
 struct UniRec;
 template <>
 struct UniRec<2> {
-  static constexpr int WORDS = 2, IDX_BITS = 11, SLOT_BITS = 11;
+  static constexpr int WORDS = 2, IDX_BITS = 12, SLOT_BITS = 12;
+  static constexpr int LO = 32 - 2 * IDX_BITS;  // slot bits in word 0
 };
 template <>
 struct UniRec<3> {
@@ -84,8 +85,9 @@ CASL_HD inline void uni_pack(const uint16_t *idx, uint32_t slot, uint32_t level,
     w[1] = uni_pack_word(idx[2], idx[3], 13, slot >> 6);
     w[2] = uni_pack_word(idx[4], idx[5], 13, level);
   } else {
-    w[0] = uni_pack_word(idx[0], idx[1], 11, slot & 1023u);
-    w[1] = uni_pack_word(idx[2], idx[3], 11, (slot >> 10) | (level << 1));
+    constexpr int B = UniRec<2>::IDX_BITS, LO = UniRec<2>::LO, S = UniRec<2>::SLOT_BITS;
+    w[0] = uni_pack_word(idx[0], idx[1], B, slot & ((1u << LO) - 1u));
+    w[1] = uni_pack_word(idx[2], idx[3], B, (slot >> LO) | (level << (S - LO)));
   }
 }
 // The LDS slot of a uniform record.
@@ -94,7 +96,9 @@ CASL_HD inline int uni_slot(const uint32_t *w) {
   if constexpr (D == 3)
     return int((w[0] >> 26) | ((w[1] >> 26) << 6));
   else
-    return int((w[0] >> 22) | (((w[1] >> 22) & 1u) << 10));
+    return int((w[0] >> (2 * UniRec<2>::IDX_BITS)) |
+               (((w[1] >> (2 * UniRec<2>::IDX_BITS)) & ((1u << (UniRec<2>::SLOT_BITS - UniRec<2>::LO)) - 1u))
+                << UniRec<2>::LO));
 }
 // Unpacks a uniform record: corner indices, LDS slot, level.
 template <int D>
@@ -108,13 +112,14 @@ CASL_HD inline void uni_unpack(const uint32_t *w, uint16_t *idx, int &slot, int 
     slot = uni_slot<3>(w);
     level = int((w[2] >> 26) & 15u);
   } else {
-    constexpr uint32_t M = (1u << 11) - 1;
+    constexpr int B = UniRec<2>::IDX_BITS;
+    constexpr uint32_t M = (1u << B) - 1;
     for (int k = 0; k < 2; ++k) {
       idx[2 * k] = uint16_t(w[k] & M);
-      idx[2 * k + 1] = uint16_t((w[k] >> 11) & M);
+      idx[2 * k + 1] = uint16_t((w[k] >> B) & M);
     }
     slot = uni_slot<2>(w);
-    level = int((w[1] >> 23) & 15u);
+    level = int((w[1] >> (2 * B + UniRec<2>::SLOT_BITS - UniRec<2>::LO)) & 15u);
   }
 }
 
@@ -142,7 +147,7 @@ constexpr uint16_t HANG_NONE = 15;
 // slots) and DMAX nodes in T u H1 (LDS second-derivative slots).  The
 // defaults are build-time tunables (CASL_TILE_T2 / CASL_TILE_T3).
 #ifndef CASL_TILE_T2
-#define CASL_TILE_T2 1024
+#define CASL_TILE_T2 2048
 #endif
 #ifndef CASL_TILE_T3
 #define CASL_TILE_T3 2048

@@ -268,8 +268,8 @@ int casl_batch_info(const casl_batch_t *b, int64_t *total_nodes,
 /* Launch schedule.  nodes_per_thread 1 or 2: the kernel that gathers through
  * the grid-wide tables, block_threads in {64, 128, 256, 512};
  * nodes_per_thread 0: the kernel that runs each tree's stages tile by tile in
- * LDS (compact records), block_threads 512 (or 256 in 2D).  block_threads 0
- * picks the default for the kind. */
+ * LDS (compact records), block_threads 512 (or 256 or 1024 in 2D).
+ * block_threads 0 picks the default for the kind. */
 int casl_batch_set_schedule(casl_batch_t *b, int32_t block_threads,
                             int32_t nodes_per_thread);
 /* tiled = 1 when every tree has a tiled encoding (nodes_per_thread 0 usable);

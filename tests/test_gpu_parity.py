@@ -356,7 +356,8 @@ def batch_trees(n, seed=17):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-@pytest.mark.parametrize("schedule", [(256, 1), (512, 0)], ids=["table_kernel", "tile_kernel"])
+@pytest.mark.parametrize("schedule", [(256, 1), (512, 0), (1024, 0)],
+                         ids=["table_kernel", "tile_kernel", "tile_kernel_1024"])
 def test_batch_matches_single_tree_path(cuda, precision, schedule):
     import torch
     say(f"[gpu] batch vs single {precision} schedule {schedule}")

@@ -371,7 +371,10 @@ __device__ __forceinline__ void transverse(int D, int a, int &t1, int &t2) {
 // The record of node n in tile tl, as the host builder's encode(): corners of
 // nonzero weight, levels, walls, hanging faces and their extension words.
 // Returns the number of hanging faces (-1 when the record cannot be encoded).
-template <int D>
+// SLOTS = false: the levels, wall bits and hanging faces only (what a
+// record's group depends on), without the slot lookups and the extension
+// words; the emission encodes every record in full and flags what fails.
+template <int D, bool SLOTS = true>
 __device__ int encode(const DTab<D> &t, const TileView &v, const TileLocal<D> &loc, int64_t n, TileRec &r,
                       uint16_t *e, int *bad) {
   constexpr int F = DTab<D>::F, C = DTab<D>::C;
@@ -393,12 +396,14 @@ __device__ int encode(const DTab<D> &t, const TileView &v, const TileLocal<D> &l
       flag(bad, BAD_ENCODE);
       return -1;
     }
-    const int32_t l = loc(t.nb(f, first, n));
-    if (l < 0 || l > 65535) {
-      flag(bad, BAD_ENCODE);
-      return -1;
+    if constexpr (SLOTS) {
+      const int32_t l = loc(t.nb(f, first, n));
+      if (l < 0 || l > 65535) {
+        flag(bad, BAD_ENCODE);
+        return -1;
+      }
+      r.idx[f] = uint16_t(l);
     }
-    r.idx[f] = uint16_t(l);
     if (nz != 1) {
       if (k == NH) {
         flag(bad, BAD_ENCODE);
@@ -408,8 +413,8 @@ __device__ int encode(const DTab<D> &t, const TileView &v, const TileLocal<D> &l
     }
   }
   if constexpr (D == 2) r.idx[4] = r.idx[5] = 0;
-  if (k > 0) {
-    geo |= GEO_IRREGULAR;
+  if (k > 0) geo |= GEO_IRREGULAR;
+  if (SLOTS && k > 0) {
     for (int q = 0; q < WORDS; ++q) e[q] = 0;
     e[0] = uint16_t(D == 2 ? hf[0] : (hf[0] | (hf[1] << 4)));
     for (int q = 0; q < k; ++q) {
@@ -525,7 +530,7 @@ __global__ void __launch_bounds__(BLOCK) k_classify(DTab<D> t, TileView v, uint6
   for (int i = int(threadIdx.x); i < nT + loc.n1; i += BLOCK) {
     TileRec r;
     uint16_t e[HangExt<D>::WORDS];
-    const int nh = encode<D>(t, v, loc, loc.node(i), r, e, bad);
+    const int nh = encode<D, false>(t, v, loc, loc.node(i), r, e, bad);
     keys[p0 + i] = (uint64_t(uint32_t(tl)) << 3) | uint32_t(group_of<D>(r, nh < 0 ? 0 : nh, i >= nT));
     vals[p0 + i] = p0 + i;
   }

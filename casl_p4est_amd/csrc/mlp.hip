@@ -211,12 +211,22 @@ __global__ void __launch_bounds__(256)
   gemm_store(acc, M, N, bm + wm, bn + wn, lane, bias, mask, ldm, relu, C, C16, ldc);
 }
 
-// out[e] = sum_z part[z*n + e], z = 0 .. splits-1 in order
+// out[e] = sum_z part[z*n + e], z = 0 .. splits-1 in order; eight loads are
+// issued ahead of their additions, so the chain waits on memory once per
+// eight splits (64-thread blocks spread the elements over every CU)
 __global__ void k_sum_splits(const float *__restrict__ part, int splits, int64_t n, float *__restrict__ out) {
   const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n) return;
   float s = part[e];
-  for (int z = 1; z < splits; ++z) s += part[size_t(z) * n + e];
+  int z = 1;
+  for (; z + 8 <= splits; z += 8) {
+    float v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = part[size_t(z + k) * n + e];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; z < splits; ++z) s += part[size_t(z) * n + e];
   out[e] = s;
 }
 
@@ -245,12 +255,14 @@ __global__ void k_mse_seed(const float *__restrict__ y, const float *__restrict_
   if ((threadIdx.x & 63) == 0) atomicAdd(loss, s);
 }
 
-// Column sums of an [n x m] FP32 matrix (bias gradients) in two passes with
-// a fixed order.  k_colsum_part: block b sums rows [b * COLSUM_ROWS, ...)
-// per column with coalesced row reads (consecutive threads take consecutive
-// columns; 256 / m row lanes when m < 256, added in lane order) into
-// part[b * m + c]; k_colsum_fin adds the blocks' partials in block order.
-constexpr int COLSUM_ROWS = 256;
+// Bias gradients, out[c] = sum over rows of x[r * m + c], in a fixed order.
+// k_colsum_part: block b sums rows [b * COLSUM_ROWS, ...) per column with
+// coalesced row reads (consecutive threads take consecutive columns; 256 / m
+// row lanes when m < 256, added in lane order) into part[b * m + c].
+// k_colsum_fin: one block per column; thread t adds partials t, t + 256, ...
+// in ascending order, then a fixed tree over the threads in LDS.  Short rows
+// per block and a tree at the end keep every dependent chain short.
+constexpr int COLSUM_ROWS = 32;
 __global__ void __launch_bounds__(256)
     k_colsum_part(const float *__restrict__ x, int n, int m, float *__restrict__ part) {
   __shared__ float red[256];
@@ -271,12 +283,19 @@ __global__ void __launch_bounds__(256)
     __syncthreads();
   }
 }
-__global__ void k_colsum_fin(const float *__restrict__ part, int nb, int m, float *__restrict__ out) {
-  const int c = int(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (c >= m) return;
-  float s = part[c];
-  for (int b = 1; b < nb; ++b) s += part[size_t(b) * m + c];
-  out[c] = s;
+__global__ void __launch_bounds__(256) k_colsum_fin(const float *__restrict__ part, int nb, int m,
+                                                    float *__restrict__ out) {
+  __shared__ float red[256];
+  const int c = int(blockIdx.x), t = int(threadIdx.x);
+  float s = 0.f;
+  for (int b = t; b < nb; b += 256) s += part[size_t(b) * m + c];
+  red[t] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) red[t] += red[t + w];
+    __syncthreads();
+  }
+  if (t == 0) out[c] = red[0];
 }
 
 // Adam on FP32 masters, refreshes the BF16 copy (may be null for biases)
@@ -714,7 +733,7 @@ int casl_mlp_backward(casl_mlp_t *m, const float *x, const float *t, int n, floa
       m->colpart.ensure(size_t(std::max(nb, 1)) * o);
       hipLaunchKernelGGL(k_colsum_part, dim3(unsigned(std::max(nb, 1))), dim3(256), 0, s, m->z32.p, n, o,
                          m->colpart.p);
-      hipLaunchKernelGGL(k_colsum_fin, dim3(blocks(o)), dim3(256), 0, s, m->colpart.p, std::max(nb, 1), o,
+      hipLaunchKernelGGL(k_colsum_fin, dim3(unsigned(o)), dim3(256), 0, s, m->colpart.p, std::max(nb, 1), o,
                          m->gb.p + m->offb[l]);
       HIP_CHECK(hipGetLastError());
       const uint16_t *Aprev = m->act16.p + m->offA[l];  // [n x in]
@@ -726,7 +745,8 @@ int casl_mlp_backward(casl_mlp_t *m, const float *x, const float *t, int n, floa
         m->part.ensure(size_t(sp) * in * o);
         gemm<true, false>(in, o, n, Aprev, in, dcur, o, nullptr, nullptr, 0, 0, m->part.p, nullptr, o, s, sp);
         const int64_t e = int64_t(in) * o;
-        hipLaunchKernelGGL(k_sum_splits, dim3(blocks(e)), dim3(256), 0, s, m->part.p, sp, e, m->gW.p + m->offW[l]);
+        hipLaunchKernelGGL(k_sum_splits, dim3(unsigned((e + 63) / 64)), dim3(64), 0, s, m->part.p, sp, e,
+                           m->gW.p + m->offW[l]);
         HIP_CHECK(hipGetLastError());
       }
       if (l > 0) {

@@ -31,6 +31,10 @@ HIP_FLAGS = [
     "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     # no packed FP32 instructions
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # the AMDGPU backend's instruction-level-parallelism scheduling strategy:
+    # the stage kernels measured 0.9 % (FP32) and 0.5 % (FP64) faster with it,
+    # at the same register counts and results
+    "-mllvm", "-amdgpu-sched-strategy=max-ilp",
     "-I", INCLUDE, "-I", CSRC,
 ]
 HOST_FLAGS = ["-O2", "-fPIC", "-std=c++17", "-ffp-contract=off", "-pthread", "-I", INCLUDE, "-I", CSRC]

@@ -30,6 +30,8 @@ def device_cache_set_limit(bytes_per_device):
     GPU, for reuse by its later allocations of the same size
     (casl_device_cache_set_limit); 0, the default, hands them back at once.
     For regrid loops: build a grid on the device, drop the old one, repeat."""
+    if int(bytes_per_device) < 0:
+        raise ValueError("bytes_per_device must be >= 0")
     L.call("casl_device_cache_set_limit", ctypes.c_size_t(int(bytes_per_device)))
 
 

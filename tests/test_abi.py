@@ -54,3 +54,6 @@ def test_device_cache_bookkeeping():
     C.device_cache_trim()
     C.device_cache_set_limit(0)
     assert L.lib().casl_device_cache_held(0, None) == L.CASL_ERR_INVALID_ARGUMENT
+    import pytest
+    with pytest.raises(ValueError):
+        C.device_cache_set_limit(-1)
